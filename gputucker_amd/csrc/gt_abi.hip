@@ -1,0 +1,952 @@
+// gt_abi.hip -- the C ABI (include/gputucker_hip.h): device context, the
+// partition planner and scheduler (host integer logic, as in the
+// reference), orchestration of the partition kernels, the mode update, the
+// fit, and the test hooks.
+//
+// HBM layout per context (one device, everything resident):
+//   coo_idx[k][nnz]  u32   global indices, block order (blocks ascending,
+//                          input order inside a block: tensor_manager.tpp:267-290)
+//   coo_val[nnz]     f32
+//   coo_blk[nnz]     u32   block id of each block-ordered nonzero
+//   st_idx[n][k][nnz] u32  mode-n stream: nonzeros sorted by (global row of
+//   st_val[n][nnz]   f32   mode n, block, position) -- the order in which
+//                          ComputingBC accumulates a row (update.cuh:73-99)
+//   row_ptr[n][rows_n+1] u64  global CSR of the stream (rows_n = P_n * bd_n)
+//   where[n][nnz]    u32   per-block where_nnz of mode n, blocks concatenated
+//   A[n]             f32   (P_n * bd_n) x R factor, padding rows zero
+//   Gt[n]            f32   mode-n unfolded core [o][jL][r]
+//   gram             f64   rows x (R(R+1)/2 + R) packed Gram of the current mode
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "gt_internal.hpp"
+
+using namespace gt;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <typename T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    reset();
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw HipError(std::string("hipMalloc(") + std::to_string(bytes) + " bytes): " +
+                     hipGetErrorString(e));
+    }
+    n = count;
+  }
+};
+
+struct GtError : std::runtime_error {
+  int code;
+  GtError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+int ceil_log2(uint64_t v) { // bits needed to hold values < v
+  int b = 0;
+  while (b < 64 && (1ull << b) < v) ++b;
+  return b;
+}
+
+// ------------------------------------------------ planner (optimizer.tpp)
+struct Plan {
+  uint32_t part[GT_MAX_ORDER], bd[GT_MAX_ORDER];
+  uint64_t blocks = 1, avg_nnz = 0;
+  int streams = 1, type = 1;
+  uint64_t avail = 0;
+};
+
+bool refresh_block_dims(int N, const uint32_t *dims, uint64_t nnz, Plan &p) {
+  p.blocks = 1;
+  for (int a = 0; a < N; ++a) { // optimizer.tpp:359-376
+    if (p.part[a] == 0) return false;
+    uint32_t bd = (uint32_t)(((uint64_t)dims[a] + p.part[a] - 1) / p.part[a]);
+    if (bd == 0) return false;
+    if ((uint32_t)(((uint64_t)dims[a] + bd - 1) / bd) != p.part[a]) return false;
+    p.bd[a] = bd;
+    p.blocks *= p.part[a];
+  }
+  p.avg_nnz = (nnz + p.blocks - 1) / p.blocks;
+  return true;
+}
+
+struct SizeModel { // optimizer.tpp:202-335
+  int N;
+  const uint32_t *dims;
+  uint32_t R;
+  uint64_t vs;
+  uint64_t core() const {
+    return (uint64_t)std::pow((double)R, (double)N) * (uint64_t)(N * 4 + vs);
+  }
+  uint64_t all_factors() const {
+    uint64_t s = 0;
+    for (int a = 0; a < N; ++a) s += (uint32_t)(dims[a] * R); // index_t product
+    return s * vs;
+  }
+  uint64_t sub_factors(const Plan &p) const {
+    uint64_t s = 0;
+    for (int a = 0; a < N; ++a) s += (uint32_t)(p.bd[a] * R);
+    return s * vs;
+  }
+  uint64_t all_data(const Plan &p) const {
+    return p.avg_nnz * (uint64_t)(N * 4 + vs) + core() + sub_factors(p) + p.avg_nnz * R * vs;
+  }
+};
+
+// FindPartitionParms (optimizer.tpp:70-164) for one GPU; forced: P given.
+bool plan_partition(int N, const uint32_t *dims, uint64_t nnz, int rank, uint64_t mem, int vs,
+                    const uint32_t *forced, Plan &p) {
+  SizeModel m{N, dims, (uint32_t)rank, (uint64_t)vs};
+  bool is_forced = false;
+  for (int a = 0; a < N; ++a) is_forced |= forced && forced[a] != 0;
+  for (int a = 0; a < N; ++a) p.part[a] = is_forced ? forced[a] : 1;
+  if (!refresh_block_dims(N, dims, nnz, p)) return false;
+  if (is_forced) {
+    p.type = (p.blocks == 1) ? 1 : 0;
+    p.streams = (p.type == 1) ? 1 : 4;
+  } else {
+    uint64_t required = nnz * (uint64_t)(N * 4 + vs) + m.core() + m.all_factors() +
+                        nnz * (uint64_t)rank * (uint64_t)vs;
+    if (required <= mem) {
+      p.type = 1;
+      p.streams = 1;
+    } else {
+      p.type = 0;
+      p.streams = 4;
+      const uint64_t buf = mem / 4;
+      do {
+        int ax = 0;
+        for (int a = 1; a < N; ++a)
+          if (p.bd[ax] < p.bd[a]) ax = a;
+        p.part[ax]++;
+        if (!refresh_block_dims(N, dims, nnz, p)) return false;
+      } while (!(m.all_data(p) < buf && p.blocks >= 4));
+    }
+  }
+  const uint64_t buf = mem / (uint64_t)p.streams;
+  const uint64_t avail_buf = buf - (m.core() + m.sub_factors(p));
+  p.avail = avail_buf / (uint64_t)(N * 4 + vs + rank * vs);
+  return true;
+}
+
+} // namespace
+
+// ------------------------------------------------ context
+struct gt_ctx {
+  gt_opts o{};
+  std::string err;
+  int N = 0, R = 0;
+  hipStream_t s = nullptr;
+  // input
+  uint64_t nnz = 0;
+  uint32_t dims[GT_MAX_ORDER] = {};
+  bool have_coo = false;
+  DevBuf<uint32_t> in_idx[GT_MAX_ORDER];
+  DevBuf<float> in_val;
+  // partition
+  bool partitioned = false;
+  Plan plan;
+  uint64_t B = 0;
+  std::vector<uint64_t> block_nnz, block_off;
+  double norm = 0.0;
+  double ms_partition = 0.0;
+  DevBuf<uint64_t> d_block_off;
+  DevBuf<uint32_t> coo_idx[GT_MAX_ORDER];
+  DevBuf<float> coo_val;
+  DevBuf<uint32_t> coo_blk;
+  DevBuf<uint32_t> st_idx[GT_MAX_ORDER][GT_MAX_ORDER];
+  DevBuf<float> st_val[GT_MAX_ORDER];
+  DevBuf<uint64_t> row_ptr[GT_MAX_ORDER];
+  DevBuf<uint32_t> where[GT_MAX_ORDER];
+  uint64_t rows[GT_MAX_ORDER] = {};
+  // model
+  bool initialized = false;
+  DevBuf<float> A[GT_MAX_ORDER];
+  DevBuf<float> Gt[GT_MAX_ORDER];
+  // work
+  DevBuf<double> gram, bpart0, bpart1, err_part, scalar;
+  uint64_t chunk = 0, nchunks = 0;
+  // timing
+  hipEvent_t ev[8] = {};
+  gt_timings tm{};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending;
+  std::vector<hipEvent_t> ev_pool;
+
+  hipEvent_t get_event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    GT_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void drain_profile() {
+    for (auto &pr : prof_pending) {
+      GT_HIP(hipEventSynchronize(pr.second));
+      float ms = 0.f;
+      GT_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+      tm.gram_kernel_ms_total += ms;
+      tm.gram_kernel_launches += 1;
+      ev_pool.push_back(pr.first);
+      ev_pool.push_back(pr.second);
+    }
+    prof_pending.clear();
+  }
+  ~gt_ctx() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      for (auto &pr : prof_pending) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+      }
+      for (auto e : ev_pool) (void)hipEventDestroy(e);
+      for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+namespace {
+
+template <typename F> int guarded(gt_ctx *ctx, F &&f) {
+  if (!ctx) return GT_ERR_ARG;
+  try {
+    GT_HIP(hipSetDevice(ctx->o.device));
+    f();
+    ctx->err.clear();
+    return GT_OK;
+  } catch (const GtError &e) {
+    ctx->err = e.what();
+    return e.code;
+  } catch (const HipError &e) {
+    ctx->err = e.what();
+    return GT_ERR_HIP;
+  } catch (const std::invalid_argument &e) {
+    ctx->err = e.what();
+    return GT_ERR_UNSUPPORTED;
+  } catch (const std::exception &e) {
+    ctx->err = e.what();
+    return GT_ERR_HIP;
+  }
+}
+
+void require(bool cond, int code, const char *msg) {
+  if (!cond) throw GtError(code, msg);
+}
+
+uint64_t ipow(uint64_t b, int e) {
+  uint64_t v = 1;
+  for (int i = 0; i < e; ++i) v *= b;
+  return v;
+}
+
+// Core unfolding for mode n in the [o][jL][r] order the kernels read.
+std::vector<float> unfold_core(const float *core, int N, int R, int n) {
+  const uint64_t total = ipow(R, N);
+  std::vector<float> gt(total);
+  const int L = (n == N - 1) ? N - 2 : N - 1;
+  int d[GT_MAX_ORDER];
+  for (uint64_t i = 0; i < total; ++i) {
+    uint64_t rem = i;
+    for (int a = N - 1; a >= 0; --a) { // tucker.cuh:120-132, last axis fastest
+      d[a] = (int)(rem % R);
+      rem /= R;
+    }
+    uint64_t o = 0;
+    for (int a = 0; a < N; ++a)
+      if (a != n && a != L) o = o * R + d[a];
+    gt[(o * R + d[L]) * R + d[n]] = core[i];
+  }
+  return gt;
+}
+
+void ensure_work(gt_ctx *c) {
+  uint64_t max_rows = 0;
+  for (int n = 0; n < c->N; ++n) max_rows = std::max(max_rows, c->rows[n]);
+  const int W = gram_width(c->R);
+  c->gram.alloc(max_rows * W);
+  uint64_t tiles = (uint64_t)c->o.chunk_tiles;
+  if (tiles == 0) { // enough chunks to give every SIMD several waves
+    const uint64_t target_waves = 16384;
+    tiles = c->nnz / (kTile * target_waves);
+    tiles = std::min<uint64_t>(std::max<uint64_t>(tiles, 1), 16);
+  }
+  c->chunk = tiles * kTile;
+  c->nchunks = (c->nnz + c->chunk - 1) / c->chunk;
+  c->bpart0.alloc(std::max<uint64_t>(c->nchunks, 1) * W);
+  c->bpart1.alloc(std::max<uint64_t>(c->nchunks, 1) * W);
+  c->err_part.alloc(4096);
+  c->scalar.alloc(16);
+}
+
+GramArgs gram_args(gt_ctx *c, int n) {
+  GramArgs a{};
+  a.order = c->N;
+  a.mode = n;
+  for (int k = 0; k < c->N; ++k) {
+    a.st.idx[k] = c->st_idx[n][k].p;
+    a.A[k] = c->A[k].p;
+  }
+  a.st.val = c->st_val[n].p;
+  a.st.nnz = c->nnz;
+  a.Gt = c->Gt[n].p;
+  a.row_ptr = c->row_ptr[n].p;
+  a.gram = c->gram.p;
+  a.bpart0 = c->bpart0.p;
+  a.bpart1 = c->bpart1.p;
+  a.chunk = c->chunk;
+  a.nchunks = c->nchunks;
+  return a;
+}
+
+void run_gram(gt_ctx *c, int n) {
+  GramArgs a = gram_args(c, n);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (c->o.profile) {
+    e0 = c->get_event();
+    e1 = c->get_event();
+    GT_HIP(hipEventRecord(e0, c->s));
+  }
+  launch_gram_fp32(c->R, a, c->s);
+  if (c->o.profile) {
+    GT_HIP(hipEventRecord(e1, c->s));
+    c->prof_pending.emplace_back(e0, e1);
+  }
+  launch_gram_fixup(c->R, a, c->s);
+}
+
+void mode_update(gt_ctx *c, int n) {
+  require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
+  require(n >= 0 && n < c->N, GT_ERR_ARG, "mode out of range");
+  if (c->prof_pending.size() > 64) c->drain_profile();
+  GT_HIP(hipEventRecord(c->ev[0], c->s));
+  run_gram(c, n);
+  GT_HIP(hipEventRecord(c->ev[1], c->s));
+  SolveArgs sa{};
+  sa.gram = c->gram.p;
+  sa.row_ptr = c->row_ptr[n].p;
+  sa.rows = c->rows[n];
+  sa.lambda = c->o.lambda;
+  sa.A = c->A[n].p;
+  launch_solve(c->R, sa, c->s);
+  GT_HIP(hipEventRecord(c->ev[2], c->s));
+  GT_HIP(hipEventSynchronize(c->ev[2]));
+  float g = 0.f, s = 0.f;
+  GT_HIP(hipEventElapsedTime(&g, c->ev[0], c->ev[1]));
+  GT_HIP(hipEventElapsedTime(&s, c->ev[1], c->ev[2]));
+  c->tm.ms_gram[n] = g;
+  c->tm.ms_solve[n] = s;
+  c->tm.ms_mode[n] = g + s;
+}
+
+CooArgs coo_args(gt_ctx *c, int mode) {
+  CooArgs a{};
+  a.order = c->N;
+  a.mode = mode;
+  a.nnz = c->nnz;
+  for (int k = 0; k < c->N; ++k) {
+    a.idx[k] = c->coo_idx[k].p;
+    a.A[k] = c->A[k].p;
+  }
+  a.val = c->coo_val.p;
+  a.Gt = c->Gt[mode].p;
+  return a;
+}
+
+double compute_fit(gt_ctx *c) {
+  require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
+  GT_HIP(hipEventRecord(c->ev[3], c->s));
+  CooArgs a = coo_args(c, c->N - 1);
+  a.err_partials = c->err_part.p;
+  int g = launch_coo_contract(c->R, a, c->s);
+  reduce_f64(c->err_part.p, g, c->scalar.p, c->s);
+  double err = 0.0;
+  GT_HIP(hipMemcpyAsync(&err, c->scalar.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
+  GT_HIP(hipEventRecord(c->ev[4], c->s));
+  GT_HIP(hipStreamSynchronize(c->s));
+  float ms = 0.f;
+  GT_HIP(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+  c->tm.ms_fit = ms;
+  if (c->norm == 0.0) return 1.0; // reconstruction.cuh:272-273
+  return 1.0 - std::sqrt(err) / c->norm;
+}
+
+void do_partition(gt_ctx *c) {
+  require(c->have_coo, GT_ERR_STATE, "gt_set_coo first");
+  const int N = c->N;
+  const uint64_t nnz = c->nnz;
+  auto t0 = std::chrono::steady_clock::now();
+  uint64_t mem = c->o.device_budget;
+  if (mem == 0) {
+    size_t fr = 0, tot = 0;
+    GT_HIP(hipMemGetInfo(&fr, &tot));
+    mem = fr;
+  }
+  Plan p;
+  if (!plan_partition(N, c->dims, nnz, c->R, mem, (int)sizeof(float), c->o.partition, p))
+    throw GtError(GT_ERR_PARTITION, "Block dimension is larger than the tensor dimension.");
+  require(p.blocks < (1ull << 32), GT_ERR_ARG, "too many blocks");
+  c->plan = p;
+  c->B = p.blocks;
+  hipStream_t s = c->s;
+  // 1. block id per nonzero, block histogram
+  DevBuf<uint32_t> keys, vals, tk, tv, counts;
+  keys.alloc(nnz);
+  vals.alloc(nnz);
+  tk.alloc(nnz);
+  tv.alloc(nnz);
+  counts.alloc(std::max<uint64_t>(c->B, 1));
+  DevBuf<char> temp;
+  uint64_t max_rows = 1;
+  for (int n = 0; n < N; ++n) max_rows = std::max<uint64_t>(max_rows, (uint64_t)p.part[n] * p.bd[n]);
+  temp.alloc(std::max(radix_sort_temp_bytes(nnz), scan_temp_bytes(std::max(nnz, max_rows) + 1)) +
+             4096);
+  fill_u32(counts.p, 0u, c->B, s);
+  PartArgs pa{};
+  pa.order = N;
+  pa.nnz = nnz;
+  for (int k = 0; k < N; ++k) {
+    pa.idx[k] = c->in_idx[k].p;
+    pa.bdims[k] = p.bd[k];
+    pa.part[k] = p.part[k];
+  }
+  block_ids(pa, keys.p, vals.p, counts.p, s);
+  std::vector<uint32_t> hcounts(c->B);
+  GT_HIP(hipMemcpyAsync(hcounts.data(), counts.p, c->B * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        s));
+  // 2. stable sort by block id: the serial in-order scatter of the reference
+  radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(c->B), temp.p, s);
+  GT_HIP(hipStreamSynchronize(s));
+  c->block_nnz.assign(c->B, 0);
+  c->block_off.assign(c->B + 1, 0);
+  for (uint64_t b = 0; b < c->B; ++b) {
+    c->block_nnz[b] = hcounts[b];
+    c->block_off[b + 1] = c->block_off[b] + hcounts[b];
+  }
+  c->d_block_off.alloc(c->B + 1);
+  GT_HIP(hipMemcpyAsync(c->d_block_off.p, c->block_off.data(), (c->B + 1) * sizeof(uint64_t),
+                        hipMemcpyHostToDevice, s));
+  // 3. block-ordered COO
+  U32Ptrs src{};
+  U32MutPtrs dst{};
+  for (int k = 0; k < N; ++k) {
+    c->coo_idx[k].alloc(nnz);
+    src.p[k] = c->in_idx[k].p;
+    dst.p[k] = c->coo_idx[k].p;
+  }
+  c->coo_val.alloc(nnz);
+  c->coo_blk.alloc(nnz);
+  gather_coo(N, nnz, vals.p, src, c->in_val.p, dst, c->coo_val.p, s);
+  GT_HIP(hipMemcpyAsync(c->coo_blk.p, keys.p, nnz * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+  // 4. norm = sqrt(sum x^2) (tensor_manager.tpp:289,293), FP64
+  DevBuf<double> sq;
+  sq.alloc(1024);
+  int g = sum_squares(c->in_val.p, nnz, sq.p, s);
+  std::vector<double> hsq(g);
+  GT_HIP(hipMemcpyAsync(hsq.data(), sq.p, g * sizeof(double), hipMemcpyDeviceToHost, s));
+  // 5. per-mode row grouping
+  for (int n = 0; n < N; ++n) {
+    const uint64_t rows = (uint64_t)p.part[n] * p.bd[n];
+    c->rows[n] = rows;
+    GT_HIP(hipMemcpyAsync(keys.p, c->coo_idx[n].p, nnz * sizeof(uint32_t),
+                          hipMemcpyDeviceToDevice, s));
+    iota_u32(vals.p, nnz, s);
+    radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(rows), temp.p, s);
+    // vals = stream order -> block-ordered position
+    U32MutPtrs sd{};
+    for (int k = 0; k < N; ++k) {
+      c->st_idx[n][k].alloc(nnz);
+      sd.p[k] = c->st_idx[n][k].p;
+    }
+    c->st_val[n].alloc(nnz);
+    U32Ptrs cs{};
+    for (int k = 0; k < N; ++k) cs.p[k] = c->coo_idx[k].p;
+    gather_coo(N, nnz, vals.p, cs, c->coo_val.p, sd, c->st_val[n].p, s);
+    // global row pointer of the stream
+    DevBuf<uint32_t> rc;
+    rc.alloc(rows);
+    fill_u32(rc.p, 0u, rows, s);
+    histogram_u32(c->st_idx[n][n].p, nnz, rc.p, s);
+    c->row_ptr[n].alloc(rows + 1);
+    exclusive_scan_u32_to_u64(rc.p, c->row_ptr[n].p, rows, temp.p, s);
+    // per-block where_nnz: stable partition of the stream order by block
+    c->where[n].alloc(nnz);
+    if (c->B == 1) {
+      GT_HIP(hipMemcpyAsync(c->where[n].p, vals.p, nnz * sizeof(uint32_t),
+                            hipMemcpyDeviceToDevice, s));
+    } else {
+      keys_from_perm(vals.p, c->coo_blk.p, nnz, keys.p, s);
+      radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(c->B), temp.p, s);
+      local_positions(vals.p, c->coo_blk.p, c->d_block_off.p, nnz, c->where[n].p, s);
+    }
+    GT_HIP(hipStreamSynchronize(s)); // rc and temp are reused
+  }
+  double ss = 0.0;
+  for (double v : hsq) ss += v;
+  c->norm = std::sqrt(ss);
+  // the raw input copy is no longer needed
+  for (int k = 0; k < N; ++k) c->in_idx[k].reset();
+  c->in_val.reset();
+  c->have_coo = false;
+  c->partitioned = true;
+  ensure_work(c);
+  c->ms_partition =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  c->tm.ms_partition = c->ms_partition;
+}
+
+} // namespace
+
+// ================================================================ C ABI
+extern "C" {
+
+int gt_version(void) { return 100; }
+
+int gt_default_opts(gt_opts *o) {
+  if (!o) return GT_ERR_ARG;
+  std::memset(o, 0, sizeof(*o));
+  o->device = 0;
+  o->order = 3;
+  o->rank = 10;                         // cmdline_opts.cpp:30
+  o->lambda = (double)0.0001f;          // constants.hpp:18 (a float literal)
+  o->max_iters = 3;                     // constants.hpp:17
+  o->tol = (double)0.0001f;             // tucker.cuh:183 compares with kLambda
+  o->precision = GT_PREC_FP32;
+  return GT_OK;
+}
+
+int gt_plan_partition(int order, const uint32_t *dims, uint64_t nnz, int rank, uint64_t mem,
+                      int value_size, const uint32_t *forced, uint32_t *part_out,
+                      uint32_t *bd_out, int *streams_out, uint64_t *avail_out, int *type_out) {
+  if (order < 1 || order > GT_MAX_ORDER || !dims || rank < 1) return GT_ERR_ARG;
+  Plan p;
+  if (!plan_partition(order, dims, nnz, rank, mem, value_size, forced, p)) return GT_ERR_PARTITION;
+  for (int a = 0; a < order; ++a) {
+    if (part_out) part_out[a] = p.part[a];
+    if (bd_out) bd_out[a] = p.bd[a];
+  }
+  if (streams_out) *streams_out = p.streams;
+  if (avail_out) *avail_out = p.avail;
+  if (type_out) *type_out = p.type;
+  return GT_OK;
+}
+
+int64_t gt_schedule(int type, int stream_count, uint64_t avail_nnz, uint64_t block_count,
+                    const uint64_t *block_nnz, uint64_t nnz_total, int64_t max_tasks,
+                    uint64_t *t_block, uint64_t *t_nnz, uint64_t *t_off, int *t_stream) {
+  // scheduler.tpp:22-159 for one GPU
+  int64_t nt = 0;
+  auto push = [&](uint64_t b, uint64_t n, uint64_t off, int st) {
+    if (nt >= max_tasks) return false;
+    t_block[nt] = b;
+    t_nnz[nt] = n;
+    t_off[nt] = off;
+    t_stream[nt] = st;
+    ++nt;
+    return true;
+  };
+  if (avail_nnz == 0) return -1;
+  if (type == 1) {
+    uint64_t task_count = 1, per = nnz_total;
+    if (per > avail_nnz) {
+      task_count = (nnz_total + avail_nnz - 1) / avail_nnz;
+      per = avail_nnz;
+    }
+    uint64_t n = per;
+    for (uint64_t t = 0; t < task_count; ++t) {
+      uint64_t ptr = t * n;
+      if (t == task_count - 1) n = nnz_total - ptr;
+      if (!push(0, n, ptr, 0)) return -1;
+    }
+    return nt;
+  }
+  std::vector<uint64_t> ids(block_count);
+  std::iota(ids.begin(), ids.end(), 0);
+  std::sort(ids.begin(), ids.end(),
+            [&](uint64_t a, uint64_t b) { return block_nnz[a] < block_nnz[b]; }); // :97
+  int g = 0;
+  for (uint64_t i = 0; i < block_count; ++i) {
+    const uint64_t b = ids[i];
+    uint64_t tasks = 1, per = block_nnz[b];
+    if (block_nnz[b] > avail_nnz) {
+      tasks = (block_nnz[b] + avail_nnz - 1) / avail_nnz;
+      per = avail_nnz;
+    }
+    uint64_t t;
+    for (t = 0; t + 1 < tasks; ++t) {
+      if (g >= stream_count) g = 0;
+      if (!push(b, per, t * per, g)) return -1;
+      ++g;
+    }
+    if (g >= stream_count) g = 0;
+    if (!push(b, block_nnz[b] - t * per, t * per, g)) return -1;
+    ++g;
+  }
+  return nt;
+}
+
+int gt_create(gt_ctx **out, const gt_opts *opts) {
+  if (!out || !opts) {
+    g_create_error = "null argument";
+    return GT_ERR_ARG;
+  }
+  *out = nullptr;
+  if (opts->order < 2 || opts->order > GT_MAX_ORDER) {
+    g_create_error = "order must be in [2, 8]";
+    return GT_ERR_UNSUPPORTED;
+  }
+  if (opts->rank < 1 || opts->rank > GT_MAX_RANK || !rank_supported(opts->rank)) {
+    g_create_error = "rank " + std::to_string(opts->rank) + " is not built";
+    return GT_ERR_UNSUPPORTED;
+  }
+  if (!(opts->lambda > 0.0)) {
+    g_create_error = "lambda must be > 0 (the row systems are solved by Cholesky)";
+    return GT_ERR_ARG;
+  }
+  if (opts->precision != GT_PREC_FP32) {
+    g_create_error = "precision not available yet";
+    return GT_ERR_UNSUPPORTED;
+  }
+  if (ipow(opts->rank, opts->order) > (1ull << 24)) {
+    g_create_error = "core too large";
+    return GT_ERR_UNSUPPORTED;
+  }
+  std::unique_ptr<gt_ctx> c(new gt_ctx);
+  c->o = *opts;
+  if (c->o.max_iters <= 0) c->o.max_iters = 3;
+  c->N = opts->order;
+  c->R = opts->rank;
+  try {
+    int count = 0;
+    GT_HIP(hipGetDeviceCount(&count));
+    if (opts->device < 0 || opts->device >= count) throw GtError(GT_ERR_ARG, "no such device");
+    GT_HIP(hipSetDevice(opts->device));
+    GT_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    for (auto &e : c->ev) GT_HIP(hipEventCreate(&e));
+  } catch (const GtError &e) {
+    g_create_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_create_error = e.what();
+    return GT_ERR_HIP;
+  }
+  *out = c.release();
+  return GT_OK;
+}
+
+int gt_destroy(gt_ctx *ctx) {
+  if (!ctx) return GT_ERR_ARG;
+  (void)hipSetDevice(ctx->o.device);
+  delete ctx;
+  return GT_OK;
+}
+
+const char *gt_last_error(const gt_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const float *vals,
+               const uint32_t *dims) {
+  return guarded(ctx, [&] {
+    require(idx && vals && dims, GT_ERR_ARG, "null argument");
+    require(nnz < (1ull << 32) - 1, GT_ERR_ARG, "nnz must be < 2^32 (uint32 positions)");
+    for (int k = 0; k < ctx->N; ++k) require(dims[k] > 0, GT_ERR_ARG, "zero dimension");
+    for (int k = 0; k < ctx->N; ++k) require(idx[k] != nullptr || nnz == 0, GT_ERR_ARG, "null idx");
+    ctx->partitioned = false;
+    ctx->initialized = false;
+    ctx->nnz = nnz;
+    for (int k = 0; k < ctx->N; ++k) {
+      ctx->dims[k] = dims[k];
+      // bounds check on the host copy (tensor_manager.tpp:237 asserts it)
+      for (uint64_t e = 0; e < nnz; ++e)
+        if (idx[k][e] >= dims[k]) throw GtError(GT_ERR_ARG, "Coordinate is out of bounds");
+      ctx->in_idx[k].alloc(nnz);
+      if (nnz)
+        GT_HIP(hipMemcpyAsync(ctx->in_idx[k].p, idx[k], nnz * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, ctx->s));
+    }
+    ctx->in_val.alloc(nnz);
+    if (nnz)
+      GT_HIP(hipMemcpyAsync(ctx->in_val.p, vals, nnz * sizeof(float), hipMemcpyHostToDevice,
+                            ctx->s));
+    GT_HIP(hipStreamSynchronize(ctx->s));
+    ctx->have_coo = true;
+  });
+}
+
+int gt_partition(gt_ctx *ctx, uint32_t *part_out, uint64_t *block_nnz_out) {
+  return guarded(ctx, [&] {
+    require(ctx->nnz > 0 || !ctx->have_coo, GT_ERR_ARG, "empty tensor");
+    do_partition(ctx);
+    if (part_out)
+      for (int k = 0; k < ctx->N; ++k) part_out[k] = ctx->plan.part[k];
+    if (block_nnz_out)
+      for (uint64_t b = 0; b < ctx->B; ++b) block_nnz_out[b] = ctx->block_nnz[b];
+  });
+}
+
+int gt_get_partition_info(gt_ctx *ctx, gt_partition_info *info) {
+  return guarded(ctx, [&] {
+    require(info != nullptr, GT_ERR_ARG, "null info");
+    require(ctx->partitioned, GT_ERR_STATE, "not partitioned");
+    std::memset(info, 0, sizeof(*info));
+    info->order = ctx->N;
+    for (int k = 0; k < ctx->N; ++k) {
+      info->dims[k] = ctx->dims[k];
+      info->partition[k] = ctx->plan.part[k];
+      info->block_dims[k] = ctx->plan.bd[k];
+    }
+    info->block_count = ctx->B;
+    info->nnz = ctx->nnz;
+    for (uint64_t b = 0; b < ctx->B; ++b) {
+      info->max_block_nnz = std::max(info->max_block_nnz, ctx->block_nnz[b]);
+      info->empty_blocks += ctx->block_nnz[b] == 0;
+    }
+    info->norm = ctx->norm;
+    info->partition_type = ctx->plan.type;
+    info->stream_count = ctx->plan.streams;
+    info->avail_nnz_per_task = ctx->plan.avail;
+  });
+}
+
+int gt_get_block_nnz(gt_ctx *ctx, uint64_t *block_nnz_out) {
+  return guarded(ctx, [&] {
+    require(ctx->partitioned, GT_ERR_STATE, "not partitioned");
+    require(block_nnz_out != nullptr, GT_ERR_ARG, "null output");
+    for (uint64_t b = 0; b < ctx->B; ++b) block_nnz_out[b] = ctx->block_nnz[b];
+  });
+}
+
+int gt_get_block(gt_ctx *ctx, uint64_t block, uint32_t *const *idx, float *vals) {
+  return guarded(ctx, [&] {
+    require(ctx->partitioned, GT_ERR_STATE, "not partitioned");
+    require(block < ctx->B, GT_ERR_ARG, "block out of range");
+    const uint64_t off = ctx->block_off[block], n = ctx->block_nnz[block];
+    if (n == 0) return;
+    uint32_t coord[GT_MAX_ORDER];
+    uint64_t rem = block;
+    for (int a = ctx->N - 1; a >= 0; --a) { // tensor.tpp:156-174
+      coord[a] = (uint32_t)(rem % ctx->plan.part[a]);
+      rem /= ctx->plan.part[a];
+    }
+    DevBuf<uint32_t> tmp;
+    tmp.alloc(n);
+    for (int k = 0; k < ctx->N; ++k) {
+      if (!idx || !idx[k]) continue;
+      sub_u32(ctx->coo_idx[k].p + off, coord[k] * ctx->plan.bd[k], n, tmp.p, ctx->s);
+      GT_HIP(hipMemcpyAsync(idx[k], tmp.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->s));
+      GT_HIP(hipStreamSynchronize(ctx->s));
+    }
+    if (vals)
+      GT_HIP(hipMemcpyAsync(vals, ctx->coo_val.p + off, n * sizeof(float), hipMemcpyDeviceToHost,
+                            ctx->s));
+    GT_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int gt_partition_dump(gt_ctx *ctx, uint64_t block, int mode, uint64_t *row_ptr, uint32_t *where) {
+  return guarded(ctx, [&] {
+    require(ctx->partitioned, GT_ERR_STATE, "not partitioned");
+    require(block < ctx->B && mode >= 0 && mode < ctx->N, GT_ERR_ARG, "block/mode out of range");
+    const uint64_t off = ctx->block_off[block], n = ctx->block_nnz[block];
+    const uint32_t bd = ctx->plan.bd[mode];
+    uint64_t rem = block, coord = 0;
+    for (int a = ctx->N - 1; a >= 0; --a) {
+      if (a == mode) coord = rem % ctx->plan.part[a];
+      rem /= ctx->plan.part[a];
+    }
+    if (where && n)
+      GT_HIP(hipMemcpyAsync(where, ctx->where[mode].p + off, n * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, ctx->s));
+    if (row_ptr) {
+      DevBuf<uint32_t> loc, cnt;
+      DevBuf<uint64_t> rp;
+      DevBuf<char> temp;
+      loc.alloc(n);
+      cnt.alloc(bd);
+      rp.alloc((uint64_t)bd + 1);
+      temp.alloc(scan_temp_bytes(bd) + 256);
+      fill_u32(cnt.p, 0u, bd, ctx->s);
+      sub_u32(ctx->coo_idx[mode].p + off, (uint32_t)(coord * bd), n, loc.p, ctx->s);
+      histogram_u32(loc.p, n, cnt.p, ctx->s);
+      exclusive_scan_u32_to_u64(cnt.p, rp.p, bd, temp.p, ctx->s);
+      GT_HIP(hipMemcpyAsync(row_ptr, rp.p, ((uint64_t)bd + 1) * sizeof(uint64_t),
+                            hipMemcpyDeviceToHost, ctx->s));
+      GT_HIP(hipStreamSynchronize(ctx->s));
+    }
+    GT_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core) {
+  return guarded(ctx, [&] {
+    require(ctx->partitioned, GT_ERR_STATE, "partition first");
+    require(A && core, GT_ERR_ARG, "null argument");
+    const int N = ctx->N, R = ctx->R;
+    for (int n = 0; n < N; ++n) {
+      require(A[n] != nullptr, GT_ERR_ARG, "null factor");
+      ctx->A[n].alloc(ctx->rows[n] * R);
+      GT_HIP(hipMemsetAsync(ctx->A[n].p, 0, ctx->rows[n] * R * sizeof(float), ctx->s));
+      GT_HIP(hipMemcpyAsync(ctx->A[n].p, A[n], (uint64_t)ctx->dims[n] * R * sizeof(float),
+                            hipMemcpyHostToDevice, ctx->s));
+      std::vector<float> gt = unfold_core(core, N, R, n);
+      ctx->Gt[n].alloc(gt.size());
+      GT_HIP(hipMemcpyAsync(ctx->Gt[n].p, gt.data(), gt.size() * sizeof(float),
+                            hipMemcpyHostToDevice, ctx->s));
+      GT_HIP(hipStreamSynchronize(ctx->s));
+    }
+    ctx->initialized = true;
+  });
+}
+
+int gt_mode_update(gt_ctx *ctx, int mode) {
+  return guarded(ctx, [&] { mode_update(ctx, mode); });
+}
+
+int gt_fit(gt_ctx *ctx, double *fit) {
+  return guarded(ctx, [&] {
+    double f = compute_fit(ctx);
+    if (fit) *fit = f;
+  });
+}
+
+int gt_iteration(gt_ctx *ctx, double *fit) {
+  return guarded(ctx, [&] {
+    for (int n = 0; n < ctx->N; ++n) mode_update(ctx, n);
+    double f = compute_fit(ctx);
+    if (fit) *fit = f;
+  });
+}
+
+int gt_run(gt_ctx *ctx, int *iters_done, double *fits) {
+  return guarded(ctx, [&] {
+    int iter = 0;
+    double p_fit = -1.0, fit = -1.0;
+    for (;;) { // tucker.cuh:166-187
+      for (int n = 0; n < ctx->N; ++n) mode_update(ctx, n);
+      fit = compute_fit(ctx);
+      if (fits) fits[iter] = fit;
+      ++iter;
+      if (iter >= ctx->o.max_iters || (p_fit != -1.0 && std::fabs(p_fit - fit) <= ctx->o.tol))
+        break;
+      p_fit = fit;
+    }
+    if (iters_done) *iters_done = iter;
+  });
+}
+
+int gt_get_factor(gt_ctx *ctx, int mode, float *out) {
+  return guarded(ctx, [&] {
+    require(ctx->initialized, GT_ERR_STATE, "not initialized");
+    require(mode >= 0 && mode < ctx->N && out, GT_ERR_ARG, "bad argument");
+    GT_HIP(hipMemcpyAsync(out, ctx->A[mode].p, (uint64_t)ctx->dims[mode] * ctx->R * sizeof(float),
+                          hipMemcpyDeviceToHost, ctx->s));
+    GT_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int gt_set_factor(gt_ctx *ctx, int mode, const float *in) {
+  return guarded(ctx, [&] {
+    require(ctx->initialized, GT_ERR_STATE, "not initialized");
+    require(mode >= 0 && mode < ctx->N && in, GT_ERR_ARG, "bad argument");
+    GT_HIP(hipMemcpyAsync(ctx->A[mode].p, in, (uint64_t)ctx->dims[mode] * ctx->R * sizeof(float),
+                          hipMemcpyHostToDevice, ctx->s));
+    GT_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int gt_get_delta(gt_ctx *ctx, int mode, float *out) {
+  return guarded(ctx, [&] {
+    require(ctx->partitioned && ctx->initialized, GT_ERR_STATE, "partition and init first");
+    require(mode >= 0 && mode < ctx->N && out, GT_ERR_ARG, "bad argument");
+    DevBuf<float> d;
+    d.alloc(ctx->nnz * ctx->R);
+    CooArgs a = coo_args(ctx, mode);
+    a.delta_out = d.p;
+    launch_coo_contract(ctx->R, a, ctx->s);
+    GT_HIP(hipMemcpyAsync(out, d.p, ctx->nnz * ctx->R * sizeof(float), hipMemcpyDeviceToHost,
+                          ctx->s));
+    GT_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int gt_get_recon(gt_ctx *ctx, float *out) {
+  return guarded(ctx, [&] {
+    require(ctx->partitioned && ctx->initialized, GT_ERR_STATE, "partition and init first");
+    require(out, GT_ERR_ARG, "bad argument");
+    DevBuf<float> x;
+    x.alloc(ctx->nnz);
+    CooArgs a = coo_args(ctx, ctx->N - 1);
+    a.xhat_out = x.p;
+    launch_coo_contract(ctx->R, a, ctx->s);
+    GT_HIP(hipMemcpyAsync(out, x.p, ctx->nnz * sizeof(float), hipMemcpyDeviceToHost, ctx->s));
+    GT_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int gt_get_gram(gt_ctx *ctx, int mode, double *B, double *C) {
+  return guarded(ctx, [&] {
+    require(ctx->partitioned && ctx->initialized, GT_ERR_STATE, "partition and init first");
+    require(mode >= 0 && mode < ctx->N && B && C, GT_ERR_ARG, "bad argument");
+    run_gram(ctx, mode);
+    const int R = ctx->R, W = gram_width(R);
+    const uint64_t rows = ctx->rows[mode];
+    std::vector<double> g(rows * W);
+    std::vector<uint64_t> rp(rows + 1);
+    GT_HIP(hipMemcpyAsync(g.data(), ctx->gram.p, rows * W * sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->s));
+    GT_HIP(hipMemcpyAsync(rp.data(), ctx->row_ptr[mode].p, (rows + 1) * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, ctx->s));
+    GT_HIP(hipStreamSynchronize(ctx->s));
+    for (uint64_t r = 0; r < rows; ++r) {
+      const bool empty = rp[r + 1] == rp[r];
+      const double *gr = g.data() + r * W;
+      int q = 0;
+      for (int p = 0; p < R; ++p)
+        for (int p2 = p; p2 < R; ++p2, ++q) {
+          const double v = empty ? 0.0 : gr[q];
+          B[(r * R + p) * R + p2] = v + (p == p2 ? ctx->o.lambda : 0.0);
+          B[(r * R + p2) * R + p] = v + (p == p2 ? ctx->o.lambda : 0.0);
+        }
+      for (int p = 0; p < R; ++p) C[r * R + p] = empty ? 0.0 : gr[R * (R + 1) / 2 + p];
+    }
+  });
+}
+
+int gt_get_timings(gt_ctx *ctx, gt_timings *t) {
+  return guarded(ctx, [&] {
+    require(t != nullptr, GT_ERR_ARG, "null");
+    ctx->drain_profile();
+    *t = ctx->tm;
+  });
+}
+
+int gt_synchronize(gt_ctx *ctx) {
+  return guarded(ctx, [&] { GT_HIP(hipStreamSynchronize(ctx->s)); });
+}
+
+} // extern "C"

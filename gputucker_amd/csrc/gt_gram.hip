@@ -1,0 +1,365 @@
+// gt_gram.hip -- the FP32 (VALU) path of one mode update: delta and the
+// per-row Gram accumulation fused in one pass over the mode's row-sorted
+// nonzero stream, the fix-up of rows that straddle chunks, and the COO-order
+// contraction used for the fit (and as a delta/xhat test hook).
+//
+// Reference: computing_delta_kernel (delta.cuh:33-92) computes
+//   delta[e][r] = sum_{j: j_n = r} G[j] prod_{k != n} A_k[i_k][j_k]
+// one thread per nonzero, and ComputingBC (update.cuh:35-101) then sums
+// B_i += delta delta^T, C_i += x delta on the host in double.  Here both
+// happen in one kernel: a wave owns a chunk of the stream (64 * tiles
+// nonzeros, rows ascending), each lane computes one nonzero's delta per tile,
+// the tile goes to LDS, and the lanes then switch roles -- lane q owns Gram
+// entry q (upper triangle of B, then C) and walks the tile's nonzeros in
+// order, flushing its FP64 accumulator whenever the row changes.  delta never
+// reaches HBM.  Products are FP32 and sums FP64, as in the reference.
+#include "gt_internal.hpp"
+
+namespace gt {
+namespace {
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// delta of mode n for this lane's nonzero.  gi[k] is the global index of mode
+// k.  The contraction runs over the other modes ascending; the last of them
+// (L) is held in registers, the others are staged in this wave's LDS rows
+// (one column per lane, so each lane only reads what it wrote).  For each
+// outer multi-index o (mixed radix over the outer modes, last fastest):
+//   s = prod_t A_t[i_t][o_t];  delta[r] += Gt[o][jL][r] * (s * A_L[i_L][jL])
+// Gt is the mode-n unfolding of the core, laid out [o][jL][r]; its address is
+// uniform across the wave, so it is read through scalar loads.
+template <int R>
+__device__ __forceinline__ void contract(int N, int n, const float *const (&A)[kMaxOrder],
+                                         const float *__restrict__ Gt,
+                                         const uint32_t (&gi)[kMaxOrder], float *lrows, int lane,
+                                         float (&d)[R]) {
+  const int L = (n == N - 1) ? N - 2 : N - 1;
+  constexpr bool kRegL = R <= 16; // above that A_L's row is staged in LDS too
+  float aL[kRegL ? R : 1];
+#pragma unroll
+  for (int k = 0; k < kMaxOrder; ++k) {
+    if (k >= N || k == n) continue;
+    const float *row = A[k] + (size_t)gi[k] * R;
+    if (kRegL && k == L) {
+#pragma unroll
+      for (int j = 0; j < (kRegL ? R : 1); ++j) aL[j] = row[j];
+    } else {
+      const int t = k - (k > n ? 1 : 0); // L lands in slot N-2
+#pragma unroll
+      for (int j = 0; j < R; ++j) lrows[(t * R + j) * kWave + lane] = row[j];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) d[r] = 0.f;
+  const int M1 = N - 2; // outer modes
+  int O = 1;
+  for (int t = 0; t < M1; ++t) O *= R;
+  int dig[kMaxOrder - 2];
+#pragma unroll
+  for (int t = 0; t < kMaxOrder - 2; ++t) dig[t] = 0;
+  for (int o = 0; o < O; ++o) {
+    float s = 1.f;
+#pragma unroll
+    for (int t = 0; t < kMaxOrder - 2; ++t)
+      if (t < M1) s *= lrows[(t * R + dig[t]) * kWave + lane];
+    const float *__restrict__ g = Gt + (size_t)o * R * R;
+    if constexpr (kRegL) {
+#pragma unroll
+      for (int jL = 0; jL < R; ++jL) {
+        const float K = s * aL[jL];
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = __builtin_fmaf(g[jL * R + r], K, d[r]);
+      }
+    } else {
+      const float *lL = lrows + (size_t)(N - 2) * R * kWave + lane;
+      for (int jL = 0; jL < R; ++jL) {
+        const float K = s * lL[jL * kWave];
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = __builtin_fmaf(g[jL * R + r], K, d[r]);
+      }
+    }
+#pragma unroll
+    for (int t = kMaxOrder - 3; t >= 0; --t) {
+      if (t < M1) {
+        if (++dig[t] < R) break;
+        dig[t] = 0;
+      }
+    }
+  }
+}
+
+// factor rows staged in LDS per lane: the outer modes, plus A_L above R = 16
+template <int R> __host__ __device__ constexpr int lds_slots(int N) {
+  return (R <= 16) ? (N > 2 ? N - 2 : 0) : N - 1;
+}
+
+template <int R> struct GramLayout {
+  static constexpr int W = gram_width(R);
+  static constexpr int E = (W + kWave - 1) / kWave;      // entries per lane
+  static constexpr int RP = (R % 2 == 0) ? R + 1 : R + 2; // odd tile pitch; column R holds x
+};
+
+template <int R>
+__global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
+  using Lay = GramLayout<R>;
+  constexpr int W = Lay::W, E = Lay::E, RP = Lay::RP;
+  __shared__ float s_tile[4][kWave * RP];
+  __shared__ uint32_t s_row[4][kWave];
+  extern __shared__ float s_rows_dyn[];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const uint64_t c = (uint64_t)blockIdx.x * 4 + w;
+  if (c >= a.nchunks) return;
+  const int N = a.order, n = a.mode;
+  float *lrows = s_rows_dyn + (size_t)w * lds_slots<R>(N) * R * kWave;
+  float *tile = s_tile[w];
+  const uint64_t cb = c * a.chunk;
+  const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
+  const uint32_t *__restrict__ rows = a.st.idx[n];
+
+  // entry ownership: q < R(R+1)/2 -> B(p, p2), p <= p2; else C(p) = delta_p * x
+  int offA[E], offB[E];
+  bool act[E];
+#pragma unroll
+  for (int qi = 0; qi < E; ++qi) {
+    const int q = lane + qi * kWave;
+    act[qi] = q < W;
+    int p = 0, p2 = 0;
+    if (q < R * (R + 1) / 2) {
+      int rem = q;
+      while (rem >= R - p) {
+        rem -= R - p;
+        ++p;
+      }
+      p2 = p + rem;
+    } else if (q < W) {
+      p = q - R * (R + 1) / 2;
+      p2 = R;
+    }
+    offA[qi] = p;
+    offB[qi] = p2;
+  }
+  double acc[E];
+#pragma unroll
+  for (int qi = 0; qi < E; ++qi) acc[qi] = 0.0;
+
+  uint32_t cur = __builtin_amdgcn_readfirstlane(rows[cb]);
+  uint64_t run_begin = cb;
+
+  auto flush = [&](uint32_t row, uint64_t b, uint64_t e_end) {
+    const uint64_t rb = a.row_ptr[row], re = a.row_ptr[row + 1];
+    double *dst;
+    if (b == rb && e_end == re)
+      dst = a.gram + (size_t)row * W;
+    else if (b == cb)
+      dst = a.bpart0 + (size_t)c * W;
+    else
+      dst = a.bpart1 + (size_t)c * W;
+#pragma unroll
+    for (int qi = 0; qi < E; ++qi)
+      if (act[qi]) dst[lane + qi * kWave] = acc[qi];
+  };
+
+  for (uint64_t tb = cb; tb < ce; tb += kWave) {
+    const uint64_t e = tb + lane;
+    const bool valid = e < ce;
+    uint32_t gi[kMaxOrder];
+#pragma unroll
+    for (int k = 0; k < kMaxOrder; ++k) gi[k] = (k < N && valid) ? a.st.idx[k][e] : 0u;
+    const float x = valid ? a.st.val[e] : 0.f;
+    float d[R];
+    contract<R>(N, n, a.A, a.Gt, gi, lrows, lane, d);
+#pragma unroll
+    for (int r = 0; r < R; ++r) tile[lane * RP + r] = d[r];
+    tile[lane * RP + R] = x;
+    s_row[w][lane] = gi[n];
+    wave_sync();
+    const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
+    for (int j = 0; j < cnt; ++j) {
+      const uint32_t row = __builtin_amdgcn_readfirstlane(s_row[w][j]);
+      if (row != cur) {
+        flush(cur, run_begin, tb + j);
+        cur = row;
+        run_begin = tb + j;
+#pragma unroll
+        for (int qi = 0; qi < E; ++qi) acc[qi] = 0.0;
+      }
+      const float *tr = tile + j * RP;
+#pragma unroll
+      for (int qi = 0; qi < E; ++qi)
+        if (act[qi]) acc[qi] += (double)(tr[offA[qi]] * tr[offB[qi]]);
+    }
+    wave_sync();
+  }
+  flush(cur, run_begin, ce);
+}
+
+// One workgroup per chunk boundary whose row continues into the next chunk
+// and starts in this chunk: sums the partials of that row over every chunk
+// it touches, in chunk order, and writes its Gram row.
+template <int R>
+__global__ void k_gram_fixup(GramArgs a) {
+  constexpr int W = gram_width(R);
+  constexpr int GROUPS = (1024 / W) > 0 ? (1024 / W) : 1;
+  __shared__ double red[GROUPS * W];
+  const uint64_t c = blockIdx.x;
+  const uint64_t nxt = (c + 1) * a.chunk;
+  if (nxt >= a.st.nnz) return;
+  const uint32_t *rows = a.st.idx[a.mode];
+  const uint32_t r = rows[nxt - 1];
+  if (rows[nxt] != r) return;
+  const uint64_t rb = a.row_ptr[r];
+  if (rb < c * a.chunk) return; // not the row's first chunk
+  const uint64_t last = (a.row_ptr[r + 1] - 1) / a.chunk;
+  const int tid = threadIdx.x;
+  const int q = tid % W, g = tid / W;
+  double s = 0.0;
+  if (g < GROUPS) {
+    if (g == 0) s = (rb == c * a.chunk ? a.bpart0 : a.bpart1)[(size_t)c * W + q];
+    for (uint64_t cc = c + 1 + g; cc <= last; cc += GROUPS) s += a.bpart0[(size_t)cc * W + q];
+    red[g * W + q] = s;
+  }
+  __syncthreads();
+  if (g == 0) {
+    double t = red[q];
+    for (int gg = 1; gg < GROUPS; ++gg) t += red[gg * W + q];
+    a.gram[(size_t)r * W + q] = t;
+  }
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) k_coo_contract(CooArgs a) {
+  __shared__ double red[256];
+  extern __shared__ float s_rows_dyn[];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int N = a.order, n = a.mode;
+  float *lrows = s_rows_dyn + (size_t)w * lds_slots<R>(N) * R * kWave;
+  double err = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < a.nnz; base += stride) {
+    const uint64_t e = base + threadIdx.x;
+    const bool valid = e < a.nnz;
+    uint32_t gi[kMaxOrder];
+#pragma unroll
+    for (int k = 0; k < kMaxOrder; ++k) gi[k] = (k < N && valid) ? a.idx[k][e] : 0u;
+    float d[R];
+    contract<R>(N, n, a.A, a.Gt, gi, lrows, lane, d);
+    if (valid) {
+      if (a.delta_out) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) a.delta_out[e * R + r] = d[r];
+      }
+      if (a.xhat_out || a.err_partials) {
+        const float *an = a.A[n] + (size_t)gi[n] * R;
+        float xh = 0.f;
+#pragma unroll
+        for (int r = 0; r < R; ++r) xh = __builtin_fmaf(an[r], d[r], xh);
+        if (a.xhat_out) a.xhat_out[e] = xh;
+        const double df = (double)a.val[e] - (double)xh;
+        err += df * df;
+      }
+    }
+  }
+  if (a.err_partials) {
+    red[threadIdx.x] = err;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) a.err_partials[blockIdx.x] = red[0];
+  }
+}
+
+__global__ void k_reduce_f64(const double *__restrict__ in, int n, double *__restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += in[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+template <int R> void gram_fp32_impl(const GramArgs &a, hipStream_t s) {
+  if (a.nchunks == 0) return;
+  const unsigned grid = (unsigned)((a.nchunks + 3) / 4);
+  const size_t lds = (size_t)4 * lds_slots<R>(a.order) * R * kWave * sizeof(float);
+  hipLaunchKernelGGL(k_gram_fp32<R>, dim3(grid), dim3(256), lds, s, a);
+}
+
+template <int R> void gram_fixup_impl(const GramArgs &a, hipStream_t s) {
+  if (a.nchunks <= 1) return;
+  constexpr int W = gram_width(R);
+  constexpr int GROUPS = (1024 / W) > 0 ? (1024 / W) : 1;
+  hipLaunchKernelGGL(k_gram_fixup<R>, dim3((unsigned)(a.nchunks - 1)), dim3(GROUPS * W), 0, s,
+                     a);
+}
+
+template <int R> int coo_impl(const CooArgs &a, hipStream_t s) {
+  uint64_t g = (a.nnz + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > 2048) g = 2048;
+  const size_t lds = (size_t)4 * lds_slots<R>(a.order) * R * kWave * sizeof(float);
+  hipLaunchKernelGGL(k_coo_contract<R>, dim3((unsigned)g), dim3(256), lds, s, a);
+  return (int)g;
+}
+
+} // namespace
+
+void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s) {
+  switch (R) {
+#define GT_CASE(r)                                                                               \
+  case r:                                                                                        \
+    gram_fp32_impl<r>(a, s);                                                                     \
+    break;
+    GT_FOR_EACH_RANK(GT_CASE)
+#undef GT_CASE
+  default:
+    throw std::invalid_argument("rank not built");
+  }
+  GT_HIP(hipGetLastError());
+}
+
+void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s) {
+  switch (R) {
+#define GT_CASE(r)                                                                               \
+  case r:                                                                                        \
+    gram_fixup_impl<r>(a, s);                                                                    \
+    break;
+    GT_FOR_EACH_RANK(GT_CASE)
+#undef GT_CASE
+  default:
+    throw std::invalid_argument("rank not built");
+  }
+  GT_HIP(hipGetLastError());
+}
+
+int launch_coo_contract(int R, const CooArgs &a, hipStream_t s) {
+  int g = 0;
+  switch (R) {
+#define GT_CASE(r)                                                                               \
+  case r:                                                                                        \
+    g = coo_impl<r>(a, s);                                                                       \
+    break;
+    GT_FOR_EACH_RANK(GT_CASE)
+#undef GT_CASE
+  default:
+    throw std::invalid_argument("rank not built");
+  }
+  GT_HIP(hipGetLastError());
+  return g;
+}
+
+void reduce_f64(const double *in, int n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_f64, dim3(1), dim3(256), 0, s, in, n, out);
+  GT_HIP(hipGetLastError());
+}
+
+} // namespace gt

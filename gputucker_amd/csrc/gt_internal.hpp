@@ -1,0 +1,151 @@
+// gt_internal.hpp -- shared declarations of the gfx950 engine: launch
+// wrappers exported by each kernel file, argument blocks, the rank list the
+// kernels are built for, and HIP error plumbing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/gputucker_hip.h"
+
+namespace gt {
+
+constexpr int kMaxOrder = GT_MAX_ORDER;
+constexpr int kWave = 64;      // CDNA wavefront width
+constexpr int kTile = 64;      // nonzeros per tile: one per lane
+
+// Packed Gram row: upper triangle of B (R(R+1)/2 doubles, row-major p <= q)
+// followed by C (R doubles).
+__host__ __device__ constexpr int gram_width(int R) { return R * (R + 1) / 2 + R; }
+
+// Ranks the templated kernels are instantiated for.  A rank outside this list
+// is rejected with GT_ERR_UNSUPPORTED by the C ABI.
+#define GT_FOR_EACH_RANK(X) \
+  X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
+  X(20) X(24) X(32)
+
+inline bool rank_supported(int R) {
+#define GT_RANK_CASE(r) if (R == r) return true;
+  GT_FOR_EACH_RANK(GT_RANK_CASE)
+#undef GT_RANK_CASE
+  return false;
+}
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string &m) : std::runtime_error(m) {}
+};
+
+#define GT_HIP(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      throw ::gt::HipError(std::string(#call) + ": " + hipGetErrorString(e_) + " at " + \
+                           __FILE__ + ":" + std::to_string(__LINE__));                   \
+  } while (0)
+
+// ---------------------------------------------------------------- sort/scan
+// Scratch bytes the radix sort of n pairs needs.
+size_t radix_sort_temp_bytes(uint64_t n);
+// Stable LSD radix sort of (key, value) pairs on bits [0, bits).  keys/vals are
+// sorted in place; tmp_keys/tmp_vals are n-element ping-pong buffers; temp is
+// radix_sort_temp_bytes(n) of scratch.
+void radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *tmp_keys, uint32_t *tmp_vals,
+                      uint64_t n, int bits, void *temp, hipStream_t s);
+size_t scan_temp_bytes(uint64_t n);
+// Exclusive scan of n uint32 counts into n+1 uint64 offsets (out[n] = total).
+void exclusive_scan_u32_to_u64(const uint32_t *in, uint64_t *out, uint64_t n, void *temp,
+                               hipStream_t s);
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+void fill_f64(double *p, double v, uint64_t n, hipStream_t s);
+void iota_u32(uint32_t *p, uint64_t n, hipStream_t s);
+
+// ---------------------------------------------------------------- partition
+struct PartArgs {
+  int order;
+  uint64_t nnz;
+  const uint32_t *idx[kMaxOrder]; // global 0-based indices, input order
+  uint32_t bdims[kMaxOrder];
+  uint32_t part[kMaxOrder];
+};
+// keys[e] = block id of nonzero e (tensor_manager.tpp:232-242), vals[e] = e.
+void block_ids(const PartArgs &a, uint32_t *keys, uint32_t *vals, uint32_t *block_counts,
+               hipStream_t s);
+struct U32Ptrs {
+  const uint32_t *p[kMaxOrder];
+};
+struct U32MutPtrs {
+  uint32_t *p[kMaxOrder];
+};
+// dst.p[k][j] = src.p[k][perm[j]] for k < order, dst_val[j] = src_val[perm[j]]
+// (dst_val may be null).
+void gather_coo(int order, uint64_t nnz, const uint32_t *perm, U32Ptrs src, const float *src_val,
+                U32MutPtrs dst, float *dst_val, hipStream_t s);
+// counts[key[e]]++ for e < n (counts must be zeroed).
+void histogram_u32(const uint32_t *key, uint64_t n, uint32_t *counts, hipStream_t s);
+// Per-block partial sums of x^2 in FP64 (deterministic order); returns the
+// number of partials written to out (<= 1024).
+int sum_squares(const float *x, uint64_t n, double *out, hipStream_t s);
+// keys[e] = block id of block-ordered position perm[e] (via blk[]), vals[e] = perm[e].
+void keys_from_perm(const uint32_t *perm, const uint32_t *blk, uint64_t n, uint32_t *keys,
+                    hipStream_t s);
+// where[j] = pos[j] - block_off[blk[pos[j]]]
+void local_positions(const uint32_t *pos, const uint32_t *blk, const uint64_t *block_off,
+                     uint64_t n, uint32_t *where, hipStream_t s);
+// out[e] = idx[e] - base  (block-local index of one block)
+void sub_u32(const uint32_t *idx, uint32_t base, uint64_t n, uint32_t *out, hipStream_t s);
+
+// ---------------------------------------------------------------- ALS kernels
+// Mode-n stream: the nonzeros sorted by (global row of mode n, block, position).
+struct StreamView {
+  const uint32_t *idx[kMaxOrder]; // global indices per mode (idx[n] is the row)
+  const float *val;
+  uint64_t nnz;
+};
+
+struct GramArgs {
+  int order;
+  int mode;
+  StreamView st;
+  const float *A[kMaxOrder];  // factors, (P_k*bd_k) x R row-major, FP32
+  const float *Gt;            // unfolded core for this mode: [R^(N-2)][R][R] ([o][jL][r])
+  const uint64_t *row_ptr;    // global CSR of this mode: rows+1
+  double *gram;               // rows x W packed Gram
+  double *bpart0;             // chunks x W: first incomplete run of each chunk
+  double *bpart1;             // chunks x W: last incomplete run of each chunk
+  uint64_t chunk;             // nonzeros per chunk (multiple of 64)
+  uint64_t nchunks;
+};
+void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s);
+void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);
+
+struct SolveArgs {
+  const double *gram;
+  const uint64_t *row_ptr;
+  uint64_t rows;
+  double lambda;
+  float *A; // rows x R, written
+};
+void launch_solve(int R, const SolveArgs &a, hipStream_t s);
+
+// Contraction over block-ordered COO: delta of `mode` (write_delta), xhat, and
+// per-workgroup FP64 partial sums of (x - xhat)^2.
+struct CooArgs {
+  int order;
+  int mode;                        // delta mode; for xhat this is order-1
+  uint64_t nnz;
+  const uint32_t *idx[kMaxOrder];  // global indices, block order
+  const float *val;
+  const float *A[kMaxOrder];
+  const float *Gt;                 // unfolded core of `mode`
+  float *delta_out;                // nnz x R or null
+  float *xhat_out;                 // nnz or null
+  double *err_partials;            // gridDim.x or null
+};
+int launch_coo_contract(int R, const CooArgs &a, hipStream_t s); // returns grid size
+void reduce_f64(const double *in, int n, double *out, hipStream_t s);
+
+} // namespace gt

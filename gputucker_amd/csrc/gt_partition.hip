@@ -1,0 +1,146 @@
+// gt_partition.hip -- the integer kernels of the tensor-block partitioner and
+// the per-mode row grouping (tensor_manager.tpp:201-302, block.tpp:114-186).
+// All of them are HBM-bound byte work: coalesced SoA reads, one pass each.
+#include "gt_internal.hpp"
+
+namespace gt {
+namespace {
+
+unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 1u << 20) {
+  uint64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// block id = sum_n floor(i_n / bd_n) * prod_{m>n} P_m (tensor_manager.tpp:232-242)
+__global__ void k_block_ids(PartArgs a, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
+                            uint32_t *__restrict__ counts) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nnz;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t id = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxOrder; ++k)
+      if (k < a.order) id = id * a.part[k] + a.idx[k][e] / a.bdims[k];
+    keys[e] = id;
+    vals[e] = (uint32_t)e;
+    atomicAdd(&counts[id], 1u);
+  }
+}
+
+__global__ void k_gather_coo(int order, uint64_t nnz, const uint32_t *__restrict__ perm,
+                             U32Ptrs src, const float *__restrict__ src_val, U32MutPtrs dst,
+                             float *__restrict__ dst_val) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = perm[j];
+    if (p >= nnz) continue; // perm is a permutation; never read outside the source
+#pragma unroll
+    for (int k = 0; k < kMaxOrder; ++k)
+      if (k < order) dst.p[k][j] = src.p[k][p];
+    if (dst_val) dst_val[j] = src_val[p];
+  }
+}
+
+__global__ void k_histogram(const uint32_t *__restrict__ key, uint64_t n,
+                            uint32_t *__restrict__ counts) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&counts[key[e]], 1u);
+}
+
+__global__ void k_sum_squares(const float *__restrict__ x, uint64_t n, double *__restrict__ out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const double v = x[e];
+    s += v * v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+__global__ void k_keys_from_perm(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ blk,
+                                 uint64_t n, uint32_t *__restrict__ keys) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    keys[e] = blk[perm[e]];
+}
+
+__global__ void k_local_positions(const uint32_t *__restrict__ pos, const uint32_t *__restrict__ blk,
+                                  const uint64_t *__restrict__ block_off, uint64_t n,
+                                  uint32_t *__restrict__ where) {
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = pos[j];
+    where[j] = (uint32_t)(p - block_off[blk[p]]);
+  }
+}
+
+__global__ void k_sub_u32(const uint32_t *__restrict__ idx, uint32_t base, uint64_t n,
+                          uint32_t *__restrict__ out) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    out[e] = idx[e] - base;
+}
+
+} // namespace
+
+void block_ids(const PartArgs &a, uint32_t *keys, uint32_t *vals, uint32_t *block_counts,
+               hipStream_t s) {
+  if (!a.nnz) return;
+  hipLaunchKernelGGL(k_block_ids, dim3(grid_for(a.nnz, 256)), dim3(256), 0, s, a, keys, vals,
+                     block_counts);
+  GT_HIP(hipGetLastError());
+}
+
+void gather_coo(int order, uint64_t nnz, const uint32_t *perm, U32Ptrs src, const float *src_val,
+                U32MutPtrs dst, float *dst_val, hipStream_t s) {
+  if (!nnz) return;
+  hipLaunchKernelGGL(k_gather_coo, dim3(grid_for(nnz, 256)), dim3(256), 0, s, order, nnz, perm,
+                     src, src_val, dst, dst_val);
+  GT_HIP(hipGetLastError());
+}
+
+void histogram_u32(const uint32_t *key, uint64_t n, uint32_t *counts, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_histogram, dim3(grid_for(n, 256)), dim3(256), 0, s, key, n, counts);
+  GT_HIP(hipGetLastError());
+}
+
+int sum_squares(const float *x, uint64_t n, double *out, hipStream_t s) {
+  const int g = (int)grid_for(n, 256, 1024);
+  hipLaunchKernelGGL(k_sum_squares, dim3(g), dim3(256), 0, s, x, n, out);
+  GT_HIP(hipGetLastError());
+  return g;
+}
+
+void keys_from_perm(const uint32_t *perm, const uint32_t *blk, uint64_t n, uint32_t *keys,
+                    hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_keys_from_perm, dim3(grid_for(n, 256)), dim3(256), 0, s, perm, blk, n,
+                     keys);
+  GT_HIP(hipGetLastError());
+}
+
+void local_positions(const uint32_t *pos, const uint32_t *blk, const uint64_t *block_off,
+                     uint64_t n, uint32_t *where, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_local_positions, dim3(grid_for(n, 256)), dim3(256), 0, s, pos, blk,
+                     block_off, n, where);
+  GT_HIP(hipGetLastError());
+}
+
+void sub_u32(const uint32_t *idx, uint32_t base, uint64_t n, uint32_t *out, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_sub_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, idx, base, n, out);
+  GT_HIP(hipGetLastError());
+}
+
+} // namespace gt

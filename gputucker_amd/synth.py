@@ -1,0 +1,121 @@
+"""Seeded synthetic sparse tensors for the benchmark workloads, and the
+reference's base-1 tab-separated coordinate text format (README.md:78-97).
+
+Coordinates are drawn independently per mode (uniform or truncated Zipf(s)
+on [0, I_n)), duplicates are dropped and more are drawn until the target
+count of distinct nonzeros is reached; the kept set is then put in a seeded
+random order, which becomes the input order.  On a GPU the draws, the dedup
+sort and the shuffle run through torch (plumbing only: the engine itself
+never uses torch); without one, numpy does the same work.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _zipf_cdf(n, s):
+    w = 1.0 / np.arange(1, n + 1, dtype=np.float64) ** s
+    c = np.cumsum(w)
+    return c / c[-1]
+
+
+def generate(dims, nnz, modes=None, values="uniform", seed=0, device=None):
+    """Returns (idx (N, nnz) uint32 0-based, vals float32, dims).
+
+    modes: per-mode "uniform" or ("zipf", s); values: "uniform" (U[0,1)) or
+    "lognormal" (mu=0, sigma=1)."""
+    dims = [int(d) for d in dims]
+    N = len(dims)
+    modes = modes or ["uniform"] * N
+    total = 1.0
+    for d in dims:
+        total *= d
+    if nnz > total:
+        raise ValueError("more nonzeros than cells")
+    if total >= 2.0 ** 63:
+        raise ValueError("index space too large for 64-bit keys")
+    strides = [1] * N
+    for k in range(N - 2, -1, -1):
+        strides[k] = strides[k + 1] * dims[k + 1]
+    use_torch = device is not None and str(device).startswith("cuda")
+    if use_torch:
+        return _generate_torch(dims, nnz, modes, values, seed, device, strides)
+    return _generate_numpy(dims, nnz, modes, values, seed, strides)
+
+
+def _generate_numpy(dims, nnz, modes, values, seed, strides):
+    rng = np.random.default_rng(seed)
+    N = len(dims)
+    cdfs = [None if m == "uniform" else _zipf_cdf(dims[k], m[1]) for k, m in enumerate(modes)]
+    keys = np.zeros(0, np.int64)
+    while len(keys) < nnz:
+        m = int((nnz - len(keys)) * 1.25) + 1024
+        key = np.zeros(m, np.int64)
+        for k in range(N):
+            if cdfs[k] is None:
+                col = rng.integers(0, dims[k], m)
+            else:
+                col = np.minimum(np.searchsorted(cdfs[k], rng.random(m), side="right"), dims[k] - 1)
+            key += col.astype(np.int64) * strides[k]
+        keys = np.unique(np.concatenate([keys, key]))
+    keys = keys[rng.permutation(len(keys))[:nnz]]
+    idx = np.zeros((N, nnz), np.uint32)
+    for k in range(N):
+        idx[k] = (keys // strides[k]) % dims[k]
+    vals = rng.random(nnz) if values == "uniform" else rng.lognormal(0.0, 1.0, nnz)
+    return idx, vals.astype(np.float32), dims
+
+
+def _generate_torch(dims, nnz, modes, values, seed, device, strides):
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    N = len(dims)
+    cdfs = [None if m == "uniform" else torch.from_numpy(_zipf_cdf(dims[k], m[1])).to(device)
+            for k, m in enumerate(modes)]
+    keys = torch.zeros(0, dtype=torch.int64, device=device)
+    while keys.numel() < nnz:
+        m = int((nnz - keys.numel()) * 1.25) + 1024
+        key = torch.zeros(m, dtype=torch.int64, device=device)
+        for k in range(N):
+            if cdfs[k] is None:
+                col = torch.randint(0, dims[k], (m,), generator=g, device=device)
+            else:
+                u = torch.rand(m, generator=g, device=device, dtype=torch.float64)
+                col = torch.searchsorted(cdfs[k], u, right=True).clamp_(max=dims[k] - 1)
+            key += col.to(torch.int64) * strides[k]
+            del col
+        keys = torch.unique(torch.cat([keys, key]))
+        del key
+    perm = torch.randperm(keys.numel(), generator=g, device=device)[:nnz]
+    keys = keys[perm]
+    del perm
+    idx = np.zeros((N, nnz), np.uint32)
+    for k in range(N):
+        idx[k] = ((keys // strides[k]) % dims[k]).to(torch.int32).cpu().numpy().view(np.uint32)
+    if values == "uniform":
+        v = torch.rand(nnz, generator=g, device=device, dtype=torch.float32)
+    else:
+        v = torch.exp(torch.randn(nnz, generator=g, device=device, dtype=torch.float32))
+    vals = v.cpu().numpy()
+    del keys, v
+    torch.cuda.empty_cache()
+    return idx, vals, dims
+
+
+def write_tns(path, idx, vals):
+    """Base-1, tab-separated: i_1 ... i_N value (README.md:78-97)."""
+    N, nnz = idx.shape
+    cols = [idx[k].astype(np.int64) + 1 for k in range(N)]
+    with open(path, "w") as f:
+        for e in range(nnz):
+            f.write("\t".join(str(int(c[e])) for c in cols) + f"\t{float(vals[e])!r}\n")
+
+
+def init_factors(dims, rank, seed=0):
+    """Factors and core drawn U[0,1) from a seed (the reference draws them
+    from glibc rand(): tucker.cuh:84-134)."""
+    rng = np.random.default_rng(seed + 7919)
+    A = [rng.random((d, rank), dtype=np.float32) for d in dims]
+    G = rng.random((rank,) * len(dims), dtype=np.float32)
+    return A, G

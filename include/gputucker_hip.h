@@ -1,0 +1,182 @@
+/*
+ * gputucker_hip.h -- C ABI of the MI355X (gfx950) sparse Tucker engine.
+ *
+ * Drop-in boundary for GPUTucker's row-wise ALS hot path.  The reference has
+ * no C/FFI layer: its boundary is the header-template API in namespace
+ * supertensor::gputucker (TuckerDecomposition, tucker.cuh:30; ParseFromFile,
+ * tensor_manager.hpp:16-17; UpdateFactorMatrices, update.cuh:128-130;
+ * Reconstruction, reconstruction.cuh:228-230).  Each entry point below names
+ * the reference routine it replaces.  The C++ driver in gputucker_amd/driver
+ * rebuilds the reference's class API and CLI on top of these calls.
+ *
+ * Conventions: every call returns GT_OK (0) or a gt_status code and never
+ * exits the process (the reference calls exit(1) on CUDA errors,
+ * cuda_helper.cpp:102-109).  gt_last_error() gives the message.  Host arrays
+ * are plain pointers owned by the caller; nothing is retained after a call
+ * returns.  A context is bound to one device and is not thread-safe.
+ * Indices are 0-based uint32 (the reference's index_t, main.cu:24); values
+ * are float (the reference's value_t in its FP32 build).
+ */
+#ifndef GPUTUCKER_HIP_H_
+#define GPUTUCKER_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_MAX_ORDER 8 /* constants.hpp:16 kMaxOrder */
+#define GT_MAX_RANK 32
+
+typedef struct gt_ctx gt_ctx;
+
+typedef enum gt_status {
+  GT_OK = 0,
+  GT_ERR_ARG = 1,         /* bad argument (the reference asserts or throws) */
+  GT_ERR_HIP = 2,         /* HIP runtime error */
+  GT_ERR_STATE = 3,       /* call out of order (e.g. mode update before partition) */
+  GT_ERR_UNSUPPORTED = 4, /* rank/order outside the built kernels */
+  GT_ERR_MEMORY = 5,      /* device budget too small */
+  GT_ERR_PARTITION = 6    /* "Block dimension is larger than the tensor dimension." */
+} gt_status;
+
+typedef enum gt_precision {
+  GT_PREC_FP32 = 0,  /* plain FP32 VALU contraction (the reference's arithmetic) */
+  GT_PREC_BF16 = 1,  /* BF16 MFMA contraction, FP32 accumulate */
+  GT_PREC_BF16R = 2  /* BF16 MFMA + split-BF16 refinement terms (near-FP32 delta) */
+} gt_precision;
+
+typedef struct gt_opts {
+  int device;                       /* HIP device ordinal */
+  int order;                        /* N, 2..GT_MAX_ORDER */
+  int rank;                         /* R per mode, 1..GT_MAX_RANK (cmdline_opts.cpp:30, default 10) */
+  double lambda;                    /* ridge term (constants.hpp:18, default (double)0.0001f) */
+  int max_iters;                    /* constants.hpp:17, default 3 */
+  double tol;                       /* fit tolerance (tucker.cuh:183, default (double)0.0001f) */
+  int precision;                    /* gt_precision */
+  uint32_t partition[GT_MAX_ORDER]; /* forced P_n; all zero = planner (optimizer.tpp:70-164) */
+  uint64_t device_budget;           /* bytes the planner may assume; 0 = free device memory */
+  int chunk_tiles;                  /* 64-nonzero tiles per Gram chunk; 0 = autotune */
+  int profile;                      /* 1: time the Gram kernel with HIP events */
+} gt_opts;
+
+typedef struct gt_partition_info {
+  int order;
+  uint32_t dims[GT_MAX_ORDER];
+  uint32_t partition[GT_MAX_ORDER];  /* P_n */
+  uint32_t block_dims[GT_MAX_ORDER]; /* ceil(I_n / P_n) */
+  uint64_t block_count;
+  uint64_t nnz;
+  uint64_t max_block_nnz;
+  uint64_t empty_blocks;
+  double norm;               /* sqrt(sum x^2), FP64 */
+  int partition_type;        /* 0 dimension-based, 1 nonzero-based (constants.hpp:38-42) */
+  int stream_count;          /* optimizer.tpp:130,140 */
+  uint64_t avail_nnz_per_task;
+} gt_partition_info;
+
+typedef struct gt_timings {
+  double ms_gram[GT_MAX_ORDER];   /* last delta+Gram phase per mode */
+  double ms_solve[GT_MAX_ORDER];  /* last row solve per mode */
+  double ms_mode[GT_MAX_ORDER];   /* last whole mode update */
+  double ms_fit;                  /* last reconstruction + fit */
+  double ms_partition;            /* partition + row grouping */
+  double gram_kernel_ms_total;    /* with opts.profile: summed event time of the Gram kernel */
+  uint64_t gram_kernel_launches;
+} gt_timings;
+
+/* Defaults as the reference compiles them in (constants.hpp:16-18,
+ * cmdline_opts.cpp:27-32). */
+int gt_default_opts(gt_opts *opts);
+
+/* Host-only partition planner, no device needed: Optimizer::FindPartitionParms
+ * and _AvailableNonzeroCountPerTask (optimizer.tpp:70-164) for one GPU with
+ * gpu_mem = mem.  forced (may be NULL or all zero) pins P_n instead of
+ * searching.  type_out: 0 dimension-based, 1 nonzero-based. */
+int gt_plan_partition(int order, const uint32_t *dims, uint64_t nnz, int rank, uint64_t mem,
+                      int value_size, const uint32_t *forced, uint32_t *part_out,
+                      uint32_t *bd_out, int *streams_out, uint64_t *avail_out, int *type_out);
+
+/* Host-only task list of Scheduler::Schedule (scheduler.tpp:22-159) for one
+ * GPU.  Returns the task count, or -1 when max_tasks is too small. */
+int64_t gt_schedule(int type, int stream_count, uint64_t avail_nnz, uint64_t block_count,
+                    const uint64_t *block_nnz, uint64_t nnz_total, int64_t max_tasks,
+                    uint64_t *t_block, uint64_t *t_nnz, uint64_t *t_off, int *t_stream);
+
+/* Replaces CudaAgent construction + AllocateMaximumBuffer (cuda_agent.tpp:20-49). */
+int gt_create(gt_ctx **out, const gt_opts *opts);
+int gt_destroy(gt_ctx *ctx);
+
+/* Last error message of ctx (or of the last failed gt_create when ctx is NULL). */
+const char *gt_last_error(const gt_ctx *ctx);
+
+/* Hands over the input tensor as _ReadData leaves it (tensor_manager.tpp:174-180):
+ * idx[n] points at nnz 0-based indices of mode n, vals at nnz values, dims at
+ * N mode sizes (the reference uses the max index seen).  Copied to HBM. */
+int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const float *vals,
+               const uint32_t *dims);
+
+/* Replaces Optimizer::FindPartitionParms (optimizer.tpp:70-89),
+ * CreateTensorBlocks (tensor_manager.tpp:201-302) and
+ * Tensor::AssignIndicesOfEachBlock (tensor.tpp:138-147), on the GPU.
+ * part_out (N entries) and block_nnz_out (block_count entries) may be NULL. */
+int gt_partition(gt_ctx *ctx, uint32_t *part_out, uint64_t *block_nnz_out);
+int gt_get_partition_info(gt_ctx *ctx, gt_partition_info *info);
+/* Nonzeros per block (block_count entries), the histogram of
+ * tensor_manager.tpp:224-256. */
+int gt_get_block_nnz(gt_ctx *ctx, uint64_t *block_nnz_out);
+
+/* Block b's COO as Block::indices/values hold it (block.hpp:166-170):
+ * idx[n] receives block_nnz local indices of mode n, vals the values. */
+int gt_get_block(gt_ctx *ctx, uint64_t block, uint32_t *const *idx, float *vals);
+
+/* Block b's row grouping for mode n as Block::count_nnz / where_nnz hold it
+ * (block.tpp:114-186): row_ptr gets block_dims[n]+1 entries, where block_nnz. */
+int gt_partition_dump(gt_ctx *ctx, uint64_t block, int mode, uint64_t *row_ptr, uint32_t *where);
+
+/* Initial factors and core (tucker.cuh:75-134): A[n] is I_n x R row-major,
+ * core is R^N values in linear order (last core axis fastest). */
+int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core);
+
+/* One mode update: delta, Gram (B = lambda I + sum dd^T, C = sum x d) and the
+ * R x R row solves, factor updated in place.  Replaces ComputingDelta
+ * (delta.cuh:116-214), ComputingBC (update.cuh:35-101) and the row-wise
+ * inverse update (update.cuh:163-194). */
+int gt_mode_update(gt_ctx *ctx, int mode);
+
+/* Replaces Reconstruction (reconstruction.cuh:228-277): fit = 1 - ||x - xhat|| / ||x||. */
+int gt_fit(gt_ctx *ctx, double *fit);
+
+/* One ALS iteration, modes 0..N-1 then the fit (update.cuh:146-198 + tucker.cuh:169-176). */
+int gt_iteration(gt_ctx *ctx, double *fit);
+
+/* The whole loop of tucker.cuh:166-187: stop at max_iters or |fit - prev| <= tol.
+ * fits receives up to max_iters values; *iters_done the count. */
+int gt_run(gt_ctx *ctx, int *iters_done, double *fits);
+
+/* Factor matrix of mode n, I_n x R row-major (the reference's
+ * factor_matrices[n][part] slices laid end to end, without padding rows). */
+int gt_get_factor(gt_ctx *ctx, int mode, float *out);
+int gt_set_factor(gt_ctx *ctx, int mode, const float *in);
+
+/* Test hooks: delta of mode n for every nonzero in block order (nnz x R),
+ * as ComputingDelta hands it back (delta.cuh:206-207). */
+int gt_get_delta(gt_ctx *ctx, int mode, float *out);
+/* Test hook: B (rows x R x R, lambda on the diagonal) and C (rows x R) of mode n
+ * over the padded row space P_n * block_dims[n], as ComputingBC leaves them. */
+int gt_get_gram(gt_ctx *ctx, int mode, double *B, double *C);
+/* Test hook: xhat for every nonzero in block order. */
+int gt_get_recon(gt_ctx *ctx, float *out);
+
+int gt_get_timings(gt_ctx *ctx, gt_timings *t);
+int gt_synchronize(gt_ctx *ctx);
+
+/* Library version: major*10000 + minor*100 + patch. */
+int gt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPUTUCKER_HIP_H_ */

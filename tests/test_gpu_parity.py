@@ -1,0 +1,165 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bars: integer work (block assignment, block COO, per-block row grouping) is
+bit-exact; delta, Gram, factors and fit are FP32-path numerics compared with
+the FP64 oracle (tolerances written at each assert)."""
+import numpy as np
+import pytest
+
+import gputucker_amd as gt
+import oracle
+from tests.helpers import random_coo, random_init
+
+pytestmark = pytest.mark.gpu
+
+
+def make(idx, vals, dims, R, part, **kw):
+    e = gt.Engine(len(dims), R, partition=part, **kw)
+    e.set_coo(idx, vals, dims)
+    e.partition()
+    return e
+
+
+def check_partition(e, t):
+    info = e.info()
+    assert info["part"].tolist() == t.part.tolist()
+    assert info["bdims"].tolist() == t.bdims.tolist()
+    assert e.block_nnz.tolist() == t.block_nnz.tolist()
+    for b in range(t.block_count):
+        lo, hi = int(t.block_off[b]), int(t.block_off[b + 1])
+        n = hi - lo
+        idx, vals = e.get_block(b, n)
+        assert np.array_equal(idx, t.lidx[:, lo:hi]), f"block {b} indices"
+        assert np.array_equal(vals, t.bvals[lo:hi].astype(np.float32)), f"block {b} values"
+        for m in range(t.order):
+            bd = int(t.bdims[m])
+            rp, wh = e.partition_dump(b, m, n, bd)
+            assert np.array_equal(rp, t.row_ptr[m][b * (bd + 1):(b + 1) * (bd + 1)]), (b, m)
+            assert np.array_equal(wh, t.where[m][lo:hi]), (b, m)
+
+
+@pytest.mark.parametrize("dims,nnz,part,skew", [
+    ([3, 2, 2], 9, [2, 1, 1], None),
+    ([40, 30, 20], 3000, [1, 1, 1], None),
+    ([40, 30, 20], 3000, [3, 2, 4], None),
+    ([500, 400, 300], 60000, [4, 4, 4], 1.1),
+    ([60, 50, 40, 30], 20000, [2, 3, 1, 2], None),
+    ([30, 30, 30, 30, 5], 30000, [2, 1, 2, 1, 1], None),
+    ([2000, 100], 150000, [7, 3], 1.3),
+])
+def test_partition_bit_exact(dims, nnz, part, skew):
+    idx, vals, dims = random_coo(len(dims) * 7 + nnz, dims, nnz, skew=skew)
+    t = oracle.Tensor(idx, vals, dims, part, 2)
+    e = make(idx, vals, dims, 2, part)
+    check_partition(e, t)
+    assert abs(e.info()["norm"] - np.sqrt(np.sum(vals.astype(np.float64) ** 2))) < 1e-9 * (
+        1 + e.info()["norm"])
+
+
+def test_partition_readme_example_and_empty_blocks():
+    from tests.golden.make_golden import read_tns
+    import os
+    idx, vals = read_tns(os.path.join(os.path.dirname(__file__), "golden", "readme_example.tns"), 3)
+    for part in ([1, 1, 1], [2, 1, 1], [3, 2, 2]):  # (3,2,2) leaves empty blocks
+        t = oracle.Tensor(idx, vals, [3, 2, 2], part, 2)
+        e = make(idx, vals, [3, 2, 2], 2, part)
+        check_partition(e, t)
+        assert abs(e.info()["norm"] ** 2 - 192.84) < 1e-4
+
+
+def _setup(seed, dims, nnz, R, part, skew=None, **kw):
+    idx, vals, dims = random_coo(seed, dims, nnz, skew=skew)
+    t = oracle.Tensor(idx, vals.astype(np.float64), dims, part, R)
+    A, G = random_init(seed, dims, R)
+    e = make(idx, vals, dims, R, part, **kw)
+    e.set_init(A, G)
+    Ap = [oracle.pad_factor(a.astype(np.float64), t.rows(k)) for k, a in enumerate(A)]
+    return e, t, Ap, G.astype(np.float64)
+
+
+@pytest.mark.parametrize("N,R,part", [(3, 10, [1, 1, 1]), (3, 10, [2, 3, 2]), (4, 8, [1, 2, 1, 1]),
+                                      (2, 5, [2, 1]), (5, 3, [1, 1, 2, 1, 1]), (3, 1, [1, 1, 1]),
+                                      (3, 16, [1, 1, 1]), (3, 20, [1, 1, 1]), (3, 32, [1, 1, 1])])
+def test_delta_parity(N, R, part):
+    dims = [50, 40, 30, 20, 10][:N]
+    e, t, A, G = _setup(N * 100 + R, dims, 4000, R, part)
+    for m in range(N):
+        d = e.get_delta(m)
+        ref = oracle.delta(t, A, G, m)
+        scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-30
+        err = np.abs(d - ref) / scale
+        assert err.max() < 2e-5 * max(1, R ** (N - 1) / 100), (m, err.max())
+
+
+@pytest.mark.parametrize("chunk_tiles", [1, 2, 0])
+def test_gram_parity_heavy_rows(chunk_tiles):
+    """Zipf rows (degrees up to thousands) with tiny chunks: many rows straddle
+    chunk boundaries and go through the fix-up."""
+    e, t, A, G = _setup(7, [300, 200, 100], 30000, 6, [2, 1, 2], skew=1.3,
+                        chunk_tiles=chunk_tiles)
+    lam = gt.default_lambda()
+    for m in range(3):
+        d = oracle.delta(t, A, G, m)
+        B, Cv = oracle.bc(t, d, m, lam)
+        Bg, Cg = e.get_gram(m, t.rows(m))
+        sc = np.abs(B).max(axis=(1, 2), keepdims=True)
+        assert (np.abs(Bg - B) / sc).max() < 1e-5
+        scc = np.abs(Cv).max(axis=1, keepdims=True) + 1e-30
+        assert (np.abs(Cg - Cv) / scc).max() < 1e-5
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+@pytest.mark.parametrize("N,R,part,skew", [(3, 10, [1, 1, 1], None), (3, 10, [2, 2, 2], 1.1),
+                                           (4, 4, [1, 2, 1, 2], None), (3, 32, [1, 1, 1], None),
+                                           (2, 7, [3, 1], None)])
+def test_mode_update_and_fit_parity(N, R, part, skew):
+    dims = [120, 100, 80, 60][:N]
+    e, t, A, G = _setup(N * 31 + R, dims, 20000, R, part, skew=skew, lam=1e-3)
+    lam = 1e-3
+    A_ref = [a.copy() for a in A]
+    for m in range(N):
+        e.mode_update(m)
+        d = oracle.delta(t, A_ref, G, m)
+        B, Cv = oracle.bc(t, d, m, lam)
+        A_ref[m] = oracle.solve(B, Cv)
+        got = e.get_factor(m)
+        assert _rel(got, A_ref[m][: dims[m]]) < 1e-3, (m, _rel(got, A_ref[m][: dims[m]]))
+        e.set_factor(m, A_ref[m][: dims[m]].astype(np.float32))  # continue from the same point
+    f = e.fit()
+    xh = oracle.recon(t, [a.astype(np.float32).astype(np.float64) for a in A_ref], G)
+    f_ref = 1 - np.sqrt(np.sum((t.bvals - xh) ** 2)) / np.sqrt(np.sum(t.vals ** 2))
+    assert abs(f - f_ref) < 1e-5, (f, f_ref)
+
+
+@pytest.mark.parametrize("part", [[1, 1, 1], [4, 4, 4]])
+def test_als_fit_trajectory_vs_oracle(part):
+    """Whole loop (gt_run): fits per iteration vs the FP64 oracle and the
+    reference-precision (FP32) oracle."""
+    dims = [200, 150, 120]
+    e, t, A, G = _setup(11, dims, 40000, 10, part, lam=1e-3, max_iters=5, tol=0.0)
+    fits = e.run()
+    f64, _ = oracle.als(t, A, G, 1e-3, 5, 0.0, "f64")
+    f32, _ = oracle.als(t, [a.astype(np.float32) for a in A], G.astype(np.float32), 1e-3, 5, 0.0,
+                        "f32")
+    assert len(fits) == 5
+    assert np.max(np.abs(fits - f64)) < 1e-5, (fits, f64)
+    assert np.max(np.abs(fits - f32)) < 1e-4, (fits, f32)
+
+
+def test_error_paths():
+    idx, vals, dims = random_coo(1, [10, 10, 10], 100)
+    with pytest.raises(gt.GtError) as ex:
+        make(idx, vals, dims, 4, [6, 1, 1])  # bd=2 -> ceil(10/2)=5 != 6
+    assert ex.value.code == 6
+    e = gt.Engine(3, 4)
+    with pytest.raises(gt.GtError) as ex:
+        e.mode_update(0)
+    assert ex.value.code == 3
+    bad = idx.copy()
+    bad[0, 0] = 10
+    with pytest.raises(gt.GtError) as ex:
+        e.set_coo(bad, vals, dims)
+    assert ex.value.code == 1
