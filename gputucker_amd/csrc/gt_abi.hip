@@ -184,6 +184,8 @@ struct gt_ctx {
   bool initialized = false;
   DevBuf<float> A[GT_MAX_ORDER];
   DevBuf<float> Gt[GT_MAX_ORDER];
+  DevBuf<uint16_t> Gb[GT_MAX_ORDER]; // BF16 core operand tables (MFMA delta)
+  int O = 1;                         // R^(N-2)
   // work
   DevBuf<double> gram, bpart0, bpart1, err_part, scalar;
   uint64_t chunk = 0, nchunks = 0;
@@ -284,6 +286,40 @@ std::vector<float> unfold_core(const float *core, int N, int R, int n) {
   return gt;
 }
 
+uint16_t to_bf16(float x) { // round to nearest even
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+float from_bf16(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+
+// B operand of the 16x16x16 BF16 MFMA for every outer index o, in lane
+// order: lane l, element t holds G[o][jL = 4(l/16)+t][r = l%16] (zero outside
+// R x R).  The hi table (BF16 of G) is followed by the lo table (BF16 of the
+// rounding remainder G - hi).
+std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R) {
+  const int O = (int)ipow(R, N > 2 ? N - 2 : 0);
+  std::vector<uint16_t> tab((size_t)2 * O * kWave * 4, 0);
+  for (int o = 0; o < O; ++o)
+    for (int l = 0; l < kWave; ++l)
+      for (int t = 0; t < 4; ++t) {
+        const int jL = 4 * (l / 16) + t, r = l % 16;
+        float g = 0.f;
+        if (jL < R && r < R) g = gt[((size_t)o * R + jL) * R + r];
+        const uint16_t hi = to_bf16(g);
+        const size_t at = ((size_t)o * kWave + l) * 4 + t;
+        tab[at] = hi;
+        tab[(size_t)O * kWave * 4 + at] = to_bf16(g - from_bf16(hi));
+      }
+  return tab;
+}
+
 void ensure_work(gt_ctx *c) {
   uint64_t max_rows = 0;
   for (int n = 0; n < c->N; ++n) max_rows = std::max(max_rows, c->rows[n]);
@@ -320,6 +356,7 @@ GramArgs gram_args(gt_ctx *c, int n) {
   a.bpart1 = c->bpart1.p;
   a.chunk = c->chunk;
   a.nchunks = c->nchunks;
+  a.Gb = c->Gb[n].p;
   return a;
 }
 
@@ -331,7 +368,10 @@ void run_gram(gt_ctx *c, int n) {
     e1 = c->get_event();
     GT_HIP(hipEventRecord(e0, c->s));
   }
-  launch_gram_fp32(c->R, a, c->s);
+  if (gram_mfma_supported(c->R))
+    launch_gram_mfma(c->R, c->o.precision, a, c->O, c->s);
+  else
+    launch_gram_fp32(c->R, a, c->s);
   if (c->o.profile) {
     GT_HIP(hipEventRecord(e1, c->s));
     c->prof_pending.emplace_back(e0, e1);
@@ -627,8 +667,12 @@ int gt_create(gt_ctx **out, const gt_opts *opts) {
     g_create_error = "lambda must be > 0 (the row systems are solved by Cholesky)";
     return GT_ERR_ARG;
   }
-  if (opts->precision != GT_PREC_FP32) {
-    g_create_error = "precision not available yet";
+  if (opts->precision < GT_PREC_FP32 || opts->precision > GT_PREC_BF16R) {
+    g_create_error = "unknown precision";
+    return GT_ERR_ARG;
+  }
+  if (opts->precision != GT_PREC_FP32 && !gram_mfma_supported(opts->rank)) {
+    g_create_error = "the BF16 MFMA delta is built for rank <= 15";
     return GT_ERR_UNSUPPORTED;
   }
   if (ipow(opts->rank, opts->order) > (1ull << 24)) {
@@ -640,6 +684,7 @@ int gt_create(gt_ctx **out, const gt_opts *opts) {
   if (c->o.max_iters <= 0) c->o.max_iters = 3;
   c->N = opts->order;
   c->R = opts->rank;
+  c->O = (int)ipow(c->R, c->N - 2);
   try {
     int count = 0;
     GT_HIP(hipGetDeviceCount(&count));
@@ -817,6 +862,12 @@ int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core) {
       ctx->Gt[n].alloc(gt.size());
       GT_HIP(hipMemcpyAsync(ctx->Gt[n].p, gt.data(), gt.size() * sizeof(float),
                             hipMemcpyHostToDevice, ctx->s));
+      if (gram_mfma_supported(R)) {
+        std::vector<uint16_t> gb = bf16_core_table(gt, N, R);
+        ctx->Gb[n].alloc(gb.size());
+        GT_HIP(hipMemcpyAsync(ctx->Gb[n].p, gb.data(), gb.size() * sizeof(uint16_t),
+                              hipMemcpyHostToDevice, ctx->s));
+      }
       GT_HIP(hipStreamSynchronize(ctx->s));
     }
     ctx->initialized = true;

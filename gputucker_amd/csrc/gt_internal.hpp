@@ -118,9 +118,14 @@ struct GramArgs {
   double *bpart1;             // chunks x W: last incomplete run of each chunk
   uint64_t chunk;             // nonzeros per chunk (multiple of 64)
   uint64_t nchunks;
+  const void *Gb;             // BF16 core operand table (hi, then lo): [o][lane][4]
 };
 void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s);
 void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);
+// delta + Gram with the Gram on MFMA (R <= 15); precision selects the delta
+// stage (GT_PREC_FP32: VALU, GT_PREC_BF16/BF16R: MFMA).  O = R^(N-2).
+bool gram_mfma_supported(int R);
+void launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
 
 struct SolveArgs {
   const double *gram;

@@ -1,0 +1,75 @@
+"""The drop-in CLI driver: flags and input-format semantics of the reference
+(cmdline_opts.cpp:27-107, tensor_manager.tpp:46-191), exercised through its
+--parse-only mode, which needs no GPU."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "gputucker_amd", "bin", "GPUTucker")
+README_TNS = os.path.join(ROOT, "tests", "golden", "readme_example.tns")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(BIN), reason="driver not built")
+
+
+def run(*args):
+    r = subprocess.run([BIN, *args], capture_output=True, text=True, timeout=60)
+    return r.returncode, r.stdout + r.stderr
+
+
+def summary(out):
+    line = [l for l in out.splitlines() if l.startswith("parsed ")][0].split()
+    order, nnz = int(line[2]), int(line[4])
+    dims = [int(x) for x in line[6:6 + order]]
+    return order, nnz, dims, float(line[-1])
+
+
+def test_readme_example_tabs():
+    rc, out = run("-i", README_TNS, "-o", "3", "--parse-only")
+    assert rc == 0, out
+    order, nnz, dims, ss = summary(out)
+    assert (order, nnz, dims) == (3, 9, [3, 2, 2])
+    assert abs(ss - 192.84) < 1e-4
+
+
+def test_readme_example_spaces(tmp_path):
+    p = tmp_path / "spaces.tns"
+    p.write_text(open(README_TNS).read().replace("\t", "    "))
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only")
+    assert rc == 0
+    assert summary(out)[:3] == (3, 9, [3, 2, 2])
+
+
+def test_zero_based_index_is_rejected(tmp_path):
+    p = tmp_path / "zero.tns"
+    p.write_text("0\t1\t1\t1.0\n1\t1\t1\t2.0\n")
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only")
+    assert rc != 0 and "base-1" in out
+
+
+def test_missing_value_draws_uniform(tmp_path):
+    p = tmp_path / "noval.tns"
+    p.write_text("1 1 1\n2 2 2 3.5\n")
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only")
+    assert rc == 0
+    order, nnz, dims, ss = summary(out)
+    assert nnz == 2 and dims == [2, 2, 2]
+    assert 12.25 <= ss <= 13.25  # 3.5^2 + u^2, u in [0, 1]
+
+
+def test_unterminated_last_line_is_not_read(tmp_path):
+    p = tmp_path / "tail.tns"
+    p.write_text("1 1 1 1.0\n2 2 2 2.0")  # the reference counts '\n'
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only")
+    assert rc == 0
+    assert summary(out)[1] == 1
+
+
+def test_option_errors(tmp_path):
+    assert run("-o", "3")[0] != 0                       # input mandatory
+    assert run("-i", README_TNS)[0] != 0                # order mandatory
+    assert run("-i", str(tmp_path), "-o", "3")[0] != 0  # not a regular file
+    rc, out = run("-h")
+    assert rc == 0 and "--input" in out and "--rank" in out and "--gpus" in out
+    assert run("-i", README_TNS, "-o", "3", "--partition", "1,2")[0] != 0

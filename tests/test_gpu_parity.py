@@ -163,3 +163,35 @@ def test_error_paths():
     with pytest.raises(gt.GtError) as ex:
         e.set_coo(bad, vals, dims)
     assert ex.value.code == 1
+
+
+@pytest.mark.parametrize("prec,tol_gram", [("bf16", 3e-2), ("bf16r", 2e-4)])
+@pytest.mark.parametrize("N,R,part", [(3, 10, [2, 1, 2]), (4, 5, [1, 2, 1, 1]), (2, 8, [1, 3])])
+def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
+    """BF16 MFMA delta (plain and with the split-BF16 refinement) feeding the
+    MFMA Gram, against the FP64 oracle's B and C."""
+    dims = [90, 70, 50, 30][:N]
+    e, t, A, G = _setup(N * 13 + R, dims, 12000, R, part, skew=1.2, precision=prec,
+                        chunk_tiles=1)
+    lam = gt.default_lambda()
+    for m in range(N):
+        d = oracle.delta(t, A, G, m)
+        B, Cv = oracle.bc(t, d, m, lam)
+        Bg, Cg = e.get_gram(m, t.rows(m))
+        sc = np.abs(B).max(axis=(1, 2), keepdims=True)
+        assert (np.abs(Bg - B) / sc).max() < tol_gram, (m, (np.abs(Bg - B) / sc).max())
+        scc = np.abs(Cv).max(axis=1, keepdims=True) + 1e-30
+        assert (np.abs(Cg - Cv) / scc).max() < tol_gram
+
+
+@pytest.mark.parametrize("prec,tol", [("bf16r", 1e-4), ("bf16", 2e-3), ("fp32", 1e-5)])
+def test_fit_vs_fp64_oracle_by_precision(prec, tol):
+    """Fit after 5 ALS iterations, relative to the FP64 oracle: the BF16 path
+    with refinement must stay within 1e-4, the FP32 path within 1e-5."""
+    dims = [200, 150, 120]
+    e, t, A, G = _setup(17, dims, 40000, 10, [2, 2, 2], skew=1.1, lam=1e-3, max_iters=5,
+                        tol=0.0, precision=prec)
+    fits = e.run()
+    f64, _ = oracle.als(t, A, G, 1e-3, 5, 0.0, "f64")
+    rel = np.abs(fits - f64) / np.abs(f64)
+    assert rel.max() < tol, (prec, fits, f64)
