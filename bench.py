@@ -84,13 +84,51 @@ def cpu_baseline(idx, vals, dims, part, R, lam, A, G, target_s=20.0):
                         f"per-nnz {per_nnz * 1e9:.1f} ns, per-mode row cost {rows_cost:.2f} s"))
 
 
+def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup, chunk_tiles):
+    """Partitions, initialises and times `steps` mode updates; returns a dict."""
+    N = len(dims)
+    e = gt.Engine(N, R, partition=part, lam=lam, precision=precision, chunk_tiles=chunk_tiles,
+                  profile=True)
+    t0 = time.perf_counter()
+    e.set_coo(idx, vals, dims)
+    e.partition()
+    e.set_init(A, G)
+    e.synchronize()
+    setup_s = time.perf_counter() - t0
+    for s in range(warmup):
+        e.mode_update(s % N)
+    e.synchronize()
+    base = e.timings()
+    it_times = []
+    t_start = time.perf_counter()
+    for it in range(steps // N):
+        ti = time.perf_counter()
+        for m in range(N):
+            e.mode_update(m)
+        e.synchronize()
+        it_times.append(time.perf_counter() - ti)
+    total = time.perf_counter() - t_start
+    tm = e.timings()
+    t_fit = time.perf_counter()
+    fit = e.fit()
+    t_fit = time.perf_counter() - t_fit
+    launches = tm["gram_kernel_launches"] - base["gram_kernel_launches"]
+    k_ms = (tm["gram_kernel_ms_total"] - base["gram_kernel_ms_total"]) / max(launches, 1)
+    t_mode = statistics.median(it_times) / N
+    e.close()
+    return dict(t_mode=t_mode, value=idx.shape[1] / t_mode, fit=fit, kernel_ms=k_ms,
+                launches=launches, total=total, setup_s=setup_s, fit_s=t_fit,
+                partition_ms=tm["ms_partition"])
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed mode updates")
+    ap.add_argument("--steps", type=int, default=None, help="timed mode updates per precision")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--precision", default="bf16r", help="headline path")
+    ap.add_argument("--compare", default="fp32", help="baseline GPU path ('' to skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk-tiles", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
@@ -115,53 +153,31 @@ def main(argv=None):
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     A, G = synth.init_factors(dims, R, seed=a.seed)
     lam = cfg["lam"] if cfg["lam"] is not None else gt.default_lambda()
-    e = gt.Engine(N, R, partition=cfg["part"], lam=lam, precision=a.precision,
-                  chunk_tiles=a.chunk_tiles, profile=True)
-    t0 = time.perf_counter()
-    e.set_coo(idx, vals, dims)
-    e.partition()
-    e.set_init(A, G)
-    e.synchronize()
-    print(f"[bench] set_coo+partition+init {time.perf_counter() - t0:.1f}s "
-          f"(partition {e.timings()['ms_partition']:.0f} ms)", file=sys.stderr, flush=True)
 
-    for s in range(warmup):
-        e.mode_update(s % N)
-    e.synchronize()
-    base = e.timings()
-    it_times = []
-    t_start = time.perf_counter()
-    for it in range(steps // N):
-        ti = time.perf_counter()
-        for m in range(N):
-            e.mode_update(m)
-        e.synchronize()
-        it_times.append(time.perf_counter() - ti)
-    e.synchronize()
-    total = time.perf_counter() - t_start
-    tm = e.timings()
-    fit = e.fit()
-    t_mode = statistics.median(it_times) / N
-    value = idx.shape[1] / t_mode
-    launches = tm["gram_kernel_launches"] - base["gram_kernel_launches"]
-    k_ms = (tm["gram_kernel_ms_total"] - base["gram_kernel_ms_total"]) / max(launches, 1)
-    s_f = 4 if a.precision == "fp32" else 2
-    bpn = bytes_per_nnz(N, R, s_f)
-    achieved = bpn * idx.shape[1] / (k_ms * 1e-3) / 1e9
-    print(f"[bench] t_mode {t_mode * 1e3:.2f} ms, gram kernel {k_ms:.2f} ms avg over {launches}, "
-          f"fit {fit:.6f}", file=sys.stderr, flush=True)
+    runs = {}
+    for prec in [p for p in (a.compare, a.precision) if p]:
+        r = run_engine(gt, idx, vals, dims, A, G, cfg["part"], R, lam, prec, steps, warmup,
+                       a.chunk_tiles)
+        runs[prec] = r
+        print(f"[bench] {prec}: t_mode {r['t_mode'] * 1e3:.3f} ms ({r['value']:.3e} nnz/s), "
+              f"gram kernel {r['kernel_ms']:.3f} ms avg over {r['launches']}, fit {r['fit']:.7f}, "
+              f"setup {r['setup_s']:.1f}s (partition {r['partition_ms']:.0f} ms), "
+              f"fit pass {r['fit_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    head = runs[a.precision]
+    bpn = bytes_per_nnz(N, R, 4)  # factor rows are gathered as FP32 on every path
+    achieved = bpn * idx.shape[1] / (head["kernel_ms"] * 1e-3) / 1e9
 
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(idx, vals, dims, cfg["part"], R, lam, A, G)
     out = {
         "metric": "nnz/s per ALS mode update",
-        "value": value,
+        "value": head["value"],
         "unit": "nnz/s",
         "n_gpus": 1,
         "steps": steps,
         "warmup": warmup,
-        "ms_per_step": t_mode * 1e3,
+        "ms_per_step": head["t_mode"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -170,13 +186,20 @@ def main(argv=None):
         "config": {"workload": cfg["name"], "nnz": int(idx.shape[1]), "order": N, "rank": R,
                    "partition": cfg["part"], "precision": a.precision,
                    "parallelism": "single GPU"},
-        "fit": fit,
-        "timed_total_s": total,
+        "fit": head["fit"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_gram_fp32", "kernel_ms": k_ms, "bytes_per_nnz": bpn},
+                     "kernel": "k_gram_mfma", "kernel_ms": head["kernel_ms"],
+                     "bytes_per_nnz": bpn},
         "cpu_baseline": cpu,
     }
+    if a.compare and a.compare in runs:
+        base = runs[a.compare]
+        out["compare"] = {"precision": a.compare, "value": base["value"],
+                          "ms_per_step": base["t_mode"] * 1e3, "fit": base["fit"],
+                          "kernel_ms": base["kernel_ms"],
+                          "speedup": head["value"] / base["value"],
+                          "fit_rel_diff": abs(head["fit"] - base["fit"]) / abs(base["fit"])}
     print(json.dumps(out))
     return 0
 

@@ -13,7 +13,8 @@
 //                          ComputingBC accumulates a row (update.cuh:73-99)
 //   row_ptr[n][rows_n+1] u64  global CSR of the stream (rows_n = P_n * bd_n)
 //   where[n][nnz]    u32   per-block where_nnz of mode n, blocks concatenated
-//   A[n]             f32   (P_n * bd_n) x R factor, padding rows zero
+//   A[n]             f32   (P_n * bd_n) x RS factor (RS = R rounded up to 4),
+//                          padding rows and columns zero
 //   Gt[n]            f32   mode-n unfolded core [o][jL][r]
 //   gram             f64   rows x (R(R+1)/2 + R) packed Gram of the current mode
 #include <algorithm>
@@ -357,6 +358,19 @@ GramArgs gram_args(gt_ctx *c, int n) {
   a.chunk = c->chunk;
   a.nchunks = c->nchunks;
   a.Gb = c->Gb[n].p;
+  // slot order: L = last mode other than n, outer slots = the rest ascending
+  const int L = (n == c->N - 1) ? c->N - 2 : c->N - 1;
+  a.slots = c->N - 2;
+  a.idx_row = c->st_idx[n][n].p;
+  a.idx_L = c->st_idx[n][L].p;
+  a.A_L = c->A[L].p;
+  int t = 0;
+  for (int k = 0; k < c->N; ++k) {
+    if (k == n || k == L) continue;
+    a.idx_O[t] = c->st_idx[n][k].p;
+    a.A_O[t] = c->A[k].p;
+    ++t;
+  }
   return a;
 }
 
@@ -403,14 +417,31 @@ void mode_update(gt_ctx *c, int n) {
   c->tm.ms_mode[n] = g + s;
 }
 
+// I_n x R host rows into the padded device layout (one contiguous copy).
+void upload_factor(gt_ctx *c, int n, const float *src) {
+  const int R = c->R, RS = row_stride(R);
+  std::vector<float> pad((size_t)c->dims[n] * RS, 0.f);
+  for (uint64_t i = 0; i < c->dims[n]; ++i) std::memcpy(pad.data() + i * RS, src + i * R, R * 4);
+  GT_HIP(hipMemcpyAsync(c->A[n].p, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice,
+                        c->s));
+  GT_HIP(hipStreamSynchronize(c->s));
+}
+
 CooArgs coo_args(gt_ctx *c, int mode) {
   CooArgs a{};
-  a.order = c->N;
-  a.mode = mode;
+  const int L = (mode == c->N - 1) ? c->N - 2 : c->N - 1;
+  a.slots = c->N - 2;
   a.nnz = c->nnz;
+  a.idx_n = c->coo_idx[mode].p;
+  a.idx_L = c->coo_idx[L].p;
+  a.A_n = c->A[mode].p;
+  a.A_L = c->A[L].p;
+  int t = 0;
   for (int k = 0; k < c->N; ++k) {
-    a.idx[k] = c->coo_idx[k].p;
-    a.A[k] = c->A[k].p;
+    if (k == mode || k == L) continue;
+    a.idx_O[t] = c->coo_idx[k].p;
+    a.A_O[t] = c->A[k].p;
+    ++t;
   }
   a.val = c->coo_val.p;
   a.Gt = c->Gt[mode].p;
@@ -474,7 +505,7 @@ void do_partition(gt_ctx *c) {
     pa.bdims[k] = p.bd[k];
     pa.part[k] = p.part[k];
   }
-  block_ids(pa, keys.p, vals.p, counts.p, s);
+  block_ids(pa, (uint32_t)c->B, keys.p, vals.p, counts.p, s);
   std::vector<uint32_t> hcounts(c->B);
   GT_HIP(hipMemcpyAsync(hcounts.data(), counts.p, c->B * sizeof(uint32_t), hipMemcpyDeviceToHost,
                         s));
@@ -526,13 +557,9 @@ void do_partition(gt_ctx *c) {
     U32Ptrs cs{};
     for (int k = 0; k < N; ++k) cs.p[k] = c->coo_idx[k].p;
     gather_coo(N, nnz, vals.p, cs, c->coo_val.p, sd, c->st_val[n].p, s);
-    // global row pointer of the stream
-    DevBuf<uint32_t> rc;
-    rc.alloc(rows);
-    fill_u32(rc.p, 0u, rows, s);
-    histogram_u32(c->st_idx[n][n].p, nnz, rc.p, s);
+    // global row pointer of the stream, from the sorted rows (keys)
     c->row_ptr[n].alloc(rows + 1);
-    exclusive_scan_u32_to_u64(rc.p, c->row_ptr[n].p, rows, temp.p, s);
+    row_ptr_from_sorted(keys.p, nnz, rows, c->row_ptr[n].p, s);
     // per-block where_nnz: stable partition of the stream order by block
     c->where[n].alloc(nnz);
     if (c->B == 1) {
@@ -543,7 +570,7 @@ void do_partition(gt_ctx *c) {
       radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(c->B), temp.p, s);
       local_positions(vals.p, c->coo_blk.p, c->d_block_off.p, nnz, c->where[n].p, s);
     }
-    GT_HIP(hipStreamSynchronize(s)); // rc and temp are reused
+    GT_HIP(hipStreamSynchronize(s)); // keys, vals and temp are reused
   }
   double ss = 0.0;
   for (double v : hsq) ss += v;
@@ -854,10 +881,10 @@ int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core) {
     const int N = ctx->N, R = ctx->R;
     for (int n = 0; n < N; ++n) {
       require(A[n] != nullptr, GT_ERR_ARG, "null factor");
-      ctx->A[n].alloc(ctx->rows[n] * R);
-      GT_HIP(hipMemsetAsync(ctx->A[n].p, 0, ctx->rows[n] * R * sizeof(float), ctx->s));
-      GT_HIP(hipMemcpyAsync(ctx->A[n].p, A[n], (uint64_t)ctx->dims[n] * R * sizeof(float),
-                            hipMemcpyHostToDevice, ctx->s));
+      const int RS = row_stride(R);
+      ctx->A[n].alloc(ctx->rows[n] * RS);
+      GT_HIP(hipMemsetAsync(ctx->A[n].p, 0, ctx->rows[n] * RS * sizeof(float), ctx->s));
+      upload_factor(ctx, n, A[n]);
       std::vector<float> gt = unfold_core(core, N, R, n);
       ctx->Gt[n].alloc(gt.size());
       GT_HIP(hipMemcpyAsync(ctx->Gt[n].p, gt.data(), gt.size() * sizeof(float),
@@ -914,9 +941,13 @@ int gt_get_factor(gt_ctx *ctx, int mode, float *out) {
   return guarded(ctx, [&] {
     require(ctx->initialized, GT_ERR_STATE, "not initialized");
     require(mode >= 0 && mode < ctx->N && out, GT_ERR_ARG, "bad argument");
-    GT_HIP(hipMemcpyAsync(out, ctx->A[mode].p, (uint64_t)ctx->dims[mode] * ctx->R * sizeof(float),
+    const int R = ctx->R, RS = row_stride(R);
+    std::vector<float> pad((size_t)ctx->dims[mode] * RS);
+    GT_HIP(hipMemcpyAsync(pad.data(), ctx->A[mode].p, pad.size() * sizeof(float),
                           hipMemcpyDeviceToHost, ctx->s));
     GT_HIP(hipStreamSynchronize(ctx->s));
+    for (uint64_t i = 0; i < ctx->dims[mode]; ++i)
+      std::memcpy(out + i * R, pad.data() + i * RS, R * sizeof(float));
   });
 }
 
@@ -924,9 +955,7 @@ int gt_set_factor(gt_ctx *ctx, int mode, const float *in) {
   return guarded(ctx, [&] {
     require(ctx->initialized, GT_ERR_STATE, "not initialized");
     require(mode >= 0 && mode < ctx->N && in, GT_ERR_ARG, "bad argument");
-    GT_HIP(hipMemcpyAsync(ctx->A[mode].p, in, (uint64_t)ctx->dims[mode] * ctx->R * sizeof(float),
-                          hipMemcpyHostToDevice, ctx->s));
-    GT_HIP(hipStreamSynchronize(ctx->s));
+    upload_factor(ctx, mode, in);
   });
 }
 

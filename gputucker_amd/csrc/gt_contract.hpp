@@ -1,10 +1,13 @@
-// gt_contract.hpp -- device helpers shared by the Gram kernels: the VALU
-// contraction that forms one nonzero's delta, and a wavefront-scope barrier.
+// gt_contract.hpp -- device helpers shared by the kernels: the VALU
+// contraction that forms one nonzero's delta, 16-byte row loads, and a
+// wavefront-scope barrier.
 #pragma once
 
 #include "gt_internal.hpp"
 
 namespace gt {
+
+constexpr int kSlots = kMaxOrder - 2;
 
 __device__ inline __attribute__((always_inline)) void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -12,77 +15,127 @@ __device__ inline __attribute__((always_inline)) void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// delta of mode n for this lane's nonzero.  gi[k] is the global index of mode
-// k.  The contraction runs over the other modes ascending; the last of them
-// (L) is held in registers, the others are staged in this wave's LDS rows
-// (one column per lane, so each lane only reads what it wrote).  For each
-// outer multi-index o (mixed radix over the outer modes, last fastest):
-//   s = prod_t A_t[i_t][o_t];  delta[r] += Gt[o][jL][r] * (s * A_L[i_L][jL])
-// Gt is the mode-n unfolding of the core, laid out [o][jL][r]; its address is
-// uniform across the wave, so it is read through scalar loads.
+typedef float gt_f4 __attribute__((ext_vector_type(4)));
+
+// row i of a factor (stride row_stride(R), zero padded) into registers with
+// 16-byte loads
 template <int R>
-__device__ __forceinline__ void contract(int N, int n, const float *const (&A)[kMaxOrder],
-                                         const float *__restrict__ Gt,
-                                         const uint32_t (&gi)[kMaxOrder], float *lrows, int lane,
-                                         float (&d)[R]) {
-  const int L = (n == N - 1) ? N - 2 : N - 1;
-  constexpr bool kRegL = R <= 16; // above that A_L's row is staged in LDS too
-  float aL[kRegL ? R : 1];
+__device__ inline __attribute__((always_inline)) void load_row(const float *__restrict__ base,
+                                                               uint32_t i,
+                                                               float (&out)[row_stride(R)]) {
+  const gt_f4 *p = reinterpret_cast<const gt_f4 *>(base + (size_t)i * row_stride(R));
 #pragma unroll
-  for (int k = 0; k < kMaxOrder; ++k) {
-    if (k >= N || k == n) continue;
-    const float *row = A[k] + (size_t)gi[k] * R;
-    if (kRegL && k == L) {
+  for (int v = 0; v < row_stride(R) / 4; ++v) {
+    const gt_f4 t = p[v];
+    out[4 * v + 0] = t[0];
+    out[4 * v + 1] = t[1];
+    out[4 * v + 2] = t[2];
+    out[4 * v + 3] = t[3];
+  }
+}
+
+// LDS floats per wave that contract_slots needs: the outer rows when there
+// are two or more outer slots (or any, above R = 16), plus A_L above R = 16.
+template <int R> __host__ __device__ constexpr int contract_lds_rows(int slots) {
+  return (R <= 16) ? (slots >= 2 ? slots : 0) : slots + 1;
+}
+
+// delta of one mode for this lane's nonzero, with the operands in slot order:
+// A_L / iL is the last mode other than n, A_O[t] / io[t] the remaining other
+// modes ascending (slots = N - 2 of them).  For each outer multi-index o
+// (mixed radix over the outer slots, last fastest):
+//   s = prod_t A_t[i_t][o_t];   delta[r] += Gt[o][jL][r] * (s * A_L[i_L][jL])
+// Gt is the mode-n unfolding of the core, [o][jL][r]; its address is uniform
+// across the wave, so it is read through scalar loads.  All gathers of the
+// lane are issued before any arithmetic.  N = 3 (one outer slot) with
+// R <= 16 is fully unrolled out of registers; otherwise rows that need a
+// runtime index are staged in this wave's LDS (one column per lane, so each
+// lane reads only what it wrote).
+template <int R>
+__device__ inline __attribute__((always_inline)) void
+contract_slots(int slots, const float *__restrict__ A_L, uint32_t iL,
+               const float *const (&A_O)[kSlots], const uint32_t (&io)[kSlots],
+               const float *__restrict__ Gt, float *lrows, int lane, float (&d)[R]) {
+  constexpr int RS = row_stride(R);
+  constexpr bool kReg = R <= 16;
+  float aL[kReg ? RS : 4];
+  float aO[kReg ? RS : 4];
+  if constexpr (kReg) {
+    load_row<R>(A_L, iL, aL);
+    if (slots == 1) load_row<R>(A_O[0], io[0], aO);
+  }
+  if (!kReg || slots >= 2) {
 #pragma unroll
-      for (int j = 0; j < (kRegL ? R : 1); ++j) aL[j] = row[j];
-    } else {
-      const int t = k - (k > n ? 1 : 0); // L lands in slot N-2
+    for (int t = 0; t < kSlots; ++t) {
+      if (t >= slots) continue;
+      const float *row = A_O[t] + (size_t)io[t] * RS;
 #pragma unroll
       for (int j = 0; j < R; ++j) lrows[(t * R + j) * kWave + lane] = row[j];
+    }
+    if (!kReg) {
+      const float *row = A_L + (size_t)iL * RS;
+#pragma unroll
+      for (int j = 0; j < R; ++j) lrows[(slots * R + j) * kWave + lane] = row[j];
     }
   }
 #pragma unroll
   for (int r = 0; r < R; ++r) d[r] = 0.f;
-  const int M1 = N - 2; // outer modes
-  int O = 1;
-  for (int t = 0; t < M1; ++t) O *= R;
-  int dig[kMaxOrder - 2];
+  if constexpr (kReg) {
+    if (slots == 0) {
 #pragma unroll
-  for (int t = 0; t < kMaxOrder - 2; ++t) dig[t] = 0;
+      for (int jL = 0; jL < R; ++jL)
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = __builtin_fmaf(Gt[jL * R + r], aL[jL], d[r]);
+      return;
+    }
+    if (slots == 1) {
+#pragma unroll
+      for (int o = 0; o < R; ++o) {
+        const float s = aO[o];
+#pragma unroll
+        for (int jL = 0; jL < R; ++jL) {
+          const float K = s * aL[jL];
+#pragma unroll
+          for (int r = 0; r < R; ++r) d[r] = __builtin_fmaf(Gt[(o * R + jL) * R + r], K, d[r]);
+        }
+      }
+      return;
+    }
+  }
+  int O = 1;
+  for (int t = 0; t < slots; ++t) O *= R;
+  int dig[kSlots];
+#pragma unroll
+  for (int t = 0; t < kSlots; ++t) dig[t] = 0;
   for (int o = 0; o < O; ++o) {
     float s = 1.f;
 #pragma unroll
-    for (int t = 0; t < kMaxOrder - 2; ++t)
-      if (t < M1) s *= lrows[(t * R + dig[t]) * kWave + lane];
-    const float *__restrict__ g = Gt + (size_t)o * R * R;
-    if constexpr (kRegL) {
+    for (int t = 0; t < kSlots; ++t)
+      if (t < slots) s *= lrows[(t * R + dig[t]) * kWave + lane];
+    const float *__restrict__ gp = Gt + (size_t)o * R * R;
+    if constexpr (kReg) {
 #pragma unroll
       for (int jL = 0; jL < R; ++jL) {
         const float K = s * aL[jL];
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = __builtin_fmaf(g[jL * R + r], K, d[r]);
+        for (int r = 0; r < R; ++r) d[r] = __builtin_fmaf(gp[jL * R + r], K, d[r]);
       }
     } else {
-      const float *lL = lrows + (size_t)(N - 2) * R * kWave + lane;
+      const float *lL = lrows + (size_t)slots * R * kWave + lane;
       for (int jL = 0; jL < R; ++jL) {
         const float K = s * lL[jL * kWave];
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r] = __builtin_fmaf(g[jL * R + r], K, d[r]);
+        for (int r = 0; r < R; ++r) d[r] = __builtin_fmaf(gp[jL * R + r], K, d[r]);
       }
     }
 #pragma unroll
-    for (int t = kMaxOrder - 3; t >= 0; --t) {
-      if (t < M1) {
+    for (int t = kSlots - 1; t >= 0; --t) {
+      if (t < slots) {
         if (++dig[t] < R) break;
         dig[t] = 0;
       }
     }
   }
-}
-
-// factor rows staged in LDS per lane: the outer modes, plus A_L above R = 16
-template <int R> __host__ __device__ constexpr int lds_slots(int N) {
-  return (R <= 16) ? (N > 2 ? N - 2 : 0) : N - 1;
 }
 
 } // namespace gt

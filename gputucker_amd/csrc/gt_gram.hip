@@ -12,7 +12,8 @@
 // the tile goes to LDS, and the lanes then switch roles -- lane q owns Gram
 // entry q (upper triangle of B, then C) and walks the tile's nonzeros in
 // order, flushing its FP64 accumulator whenever the row changes.  delta never
-// reaches HBM.  Products are FP32 and sums FP64, as in the reference.
+// reaches HBM.  Products of the FP32 deltas are formed exactly in FP64 and
+// summed in FP64 (the reference rounds each product to value_t first).
 #include "gt_contract.hpp"
 
 namespace gt {
@@ -34,12 +35,11 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const uint64_t c = (uint64_t)blockIdx.x * 4 + w;
   if (c >= a.nchunks) return;
-  const int N = a.order, n = a.mode;
-  float *lrows = s_rows_dyn + (size_t)w * lds_slots<R>(N) * R * kWave;
+  float *lrows = s_rows_dyn + (size_t)w * contract_lds_rows<R>(a.slots) * R * kWave;
   float *tile = s_tile[w];
   const uint64_t cb = c * a.chunk;
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
-  const uint32_t *__restrict__ rows = a.st.idx[n];
+  const uint32_t *__restrict__ rows = a.idx_row;
 
   // entry ownership: q < R(R+1)/2 -> B(p, p2), p <= p2; else C(p) = delta_p * x
   int offA[E], offB[E];
@@ -87,16 +87,18 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
   for (uint64_t tb = cb; tb < ce; tb += kWave) {
     const uint64_t e = tb + lane;
     const bool valid = e < ce;
-    uint32_t gi[kMaxOrder];
+    const uint64_t ee = valid ? e : cb; // in-bounds position for padding lanes
+    const uint32_t iL = a.idx_L[ee];
+    uint32_t io[kSlots];
 #pragma unroll
-    for (int k = 0; k < kMaxOrder; ++k) gi[k] = (k < N && valid) ? a.st.idx[k][e] : 0u;
-    const float x = valid ? a.st.val[e] : 0.f;
+    for (int t = 0; t < kSlots; ++t) io[t] = (t < a.slots) ? a.idx_O[t][ee] : 0u;
+    const float x = valid ? a.st.val[ee] : 0.f;
     float d[R];
-    contract<R>(N, n, a.A, a.Gt, gi, lrows, lane, d);
+    contract_slots<R>(a.slots, a.A_L, iL, a.A_O, io, a.Gt, lrows, lane, d);
 #pragma unroll
     for (int r = 0; r < R; ++r) tile[lane * RP + r] = d[r];
     tile[lane * RP + R] = x;
-    s_row[w][lane] = gi[n];
+    s_row[w][lane] = rows[ee];
     wave_sync();
     const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
     for (int j = 0; j < cnt; ++j) {
@@ -111,7 +113,7 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
       const float *tr = tile + j * RP;
 #pragma unroll
       for (int qi = 0; qi < E; ++qi)
-        if (act[qi]) acc[qi] += (double)(tr[offA[qi]] * tr[offB[qi]]);
+        if (act[qi]) acc[qi] += (double)tr[offA[qi]] * (double)tr[offB[qi]];
     }
     wave_sync();
   }
@@ -129,7 +131,7 @@ __global__ void k_gram_fixup(GramArgs a) {
   const uint64_t c = blockIdx.x;
   const uint64_t nxt = (c + 1) * a.chunk;
   if (nxt >= a.st.nnz) return;
-  const uint32_t *rows = a.st.idx[a.mode];
+  const uint32_t *rows = a.idx_row;
   const uint32_t r = rows[nxt - 1];
   if (rows[nxt] != r) return;
   const uint64_t rb = a.row_ptr[r];
@@ -156,25 +158,26 @@ __global__ void __launch_bounds__(256) k_coo_contract(CooArgs a) {
   __shared__ double red[256];
   extern __shared__ float s_rows_dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const int N = a.order, n = a.mode;
-  float *lrows = s_rows_dyn + (size_t)w * lds_slots<R>(N) * R * kWave;
+  float *lrows = s_rows_dyn + (size_t)w * contract_lds_rows<R>(a.slots) * R * kWave;
   double err = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < a.nnz; base += stride) {
     const uint64_t e = base + threadIdx.x;
     const bool valid = e < a.nnz;
-    uint32_t gi[kMaxOrder];
+    const uint64_t ee = valid ? e : 0;
+    const uint32_t iL = a.idx_L[ee];
+    uint32_t io[kSlots];
 #pragma unroll
-    for (int k = 0; k < kMaxOrder; ++k) gi[k] = (k < N && valid) ? a.idx[k][e] : 0u;
+    for (int t = 0; t < kSlots; ++t) io[t] = (t < a.slots) ? a.idx_O[t][ee] : 0u;
     float d[R];
-    contract<R>(N, n, a.A, a.Gt, gi, lrows, lane, d);
+    contract_slots<R>(a.slots, a.A_L, iL, a.A_O, io, a.Gt, lrows, lane, d);
     if (valid) {
       if (a.delta_out) {
 #pragma unroll
         for (int r = 0; r < R; ++r) a.delta_out[e * R + r] = d[r];
       }
       if (a.xhat_out || a.err_partials) {
-        const float *an = a.A[n] + (size_t)gi[n] * R;
+        const float *an = a.A_n + (size_t)a.idx_n[e] * row_stride(R);
         float xh = 0.f;
 #pragma unroll
         for (int r = 0; r < R; ++r) xh = __builtin_fmaf(an[r], d[r], xh);
@@ -211,7 +214,7 @@ __global__ void k_reduce_f64(const double *__restrict__ in, int n, double *__res
 template <int R> void gram_fp32_impl(const GramArgs &a, hipStream_t s) {
   if (a.nchunks == 0) return;
   const unsigned grid = (unsigned)((a.nchunks + 3) / 4);
-  const size_t lds = (size_t)4 * lds_slots<R>(a.order) * R * kWave * sizeof(float);
+  const size_t lds = (size_t)4 * contract_lds_rows<R>(a.slots) * R * kWave * sizeof(float);
   hipLaunchKernelGGL(k_gram_fp32<R>, dim3(grid), dim3(256), lds, s, a);
 }
 
@@ -227,7 +230,7 @@ template <int R> int coo_impl(const CooArgs &a, hipStream_t s) {
   uint64_t g = (a.nnz + 255) / 256;
   if (g < 1) g = 1;
   if (g > 2048) g = 2048;
-  const size_t lds = (size_t)4 * lds_slots<R>(a.order) * R * kWave * sizeof(float);
+  const size_t lds = (size_t)4 * contract_lds_rows<R>(a.slots) * R * kWave * sizeof(float);
   hipLaunchKernelGGL(k_coo_contract<R>, dim3((unsigned)g), dim3(256), lds, s, a);
   return (int)g;
 }

@@ -8,25 +8,36 @@
 //           a = a_hi + a_lo and G = G_hi + G_lo (each part BF16),
 //           a.G ~ a_hi.G_hi + a_lo.G_hi + a_hi.G_lo, all on MFMA
 //
-// BF16 delta, two stages per 64-nonzero tile.  Let L be the last mode other
-// than n and o the multi-index of the remaining "outer" modes:
+// Operands come in "slot order" (built on the host per mode): L is the last
+// mode other than n, the outer slots t are the remaining other modes
+// ascending, o is the mixed-radix index over the outer slots (last fastest).
+// The kernel never indexes a register array by a runtime mode number, so all
+// gathers of a tile are issued back to back.  Factor rows have a stride of
+// R rounded up to 4 floats and are read with 16-byte loads.
+//
+// VALU delta: delta[r] = sum_o s[o] sum_jL A_L[jL] G[o][jL][r], with
+// s[o] = prod_t A_t[o_t]; for N = 3 (one outer slot) the whole contraction
+// is unrolled with the core read through scalar loads.
+//
+// BF16 delta, two stages per 64-nonzero tile:
 //   stage 1 (MFMA 16x16x16 bf16):  U[m][(o,r)] = sum_jL A_L[i_L(m)][jL] G[o][jL][r]
-//   stage 2 (VALU FP32):           delta[m][r] = sum_o U[m][(o,r)] * s_m[o],
-//                                  s_m[o] = prod_t A_t[i_t(m)][o_t]
+//   stage 2 (VALU FP32):           delta[m][r] = sum_o U[m][(o,r)] * s_m[o]
 // The A operand is the FP32 factor row of each nonzero split to BF16 on the
 // fly; the B operand (the core, zero-padded to 16 x 16 per o) comes from a
 // small table laid out in lane order.
 //
-// Gram on MFMA (16x16x4 f32).  After the delta stage lane l holds
+// Gram on MFMA (16x16x4 f64).  After the delta stage lane l holds
 // v[mb][i] = delta[m][q] with m = 16 mb + 4 (l/16) + i and q = l % 16, with
 // x[m] in column q = R and zeros beyond.  For a fixed (mb, i) the four lane
 // groups hold four different nonzeros, so one MFMA with v as both operands
 // adds sum_m v[m][p] v[m][q] for those four nonzeros: the upper triangle is
 // B = sum delta delta^T, column R is C = sum x delta.  Each run of equal rows
-// inside the tile gets its own masked MFMAs; partial sums stay in FP32 for
-// at most 64 products and are then added into FP64 accumulators, which are
-// flushed on row change exactly like the VALU Gram (complete rows to the
-// Gram table, chunk-straddling runs to the partial tables).
+// inside the tile gets its own masked MFMAs.  The FP32 deltas enter as FP64,
+// so every product is exact and every sum is FP64: B is positive definite to
+// FP64 rounding even for rows with fewer than R nonzeros, where FP32-rounded
+// products would shift the zero eigenvalues of sum dd^T by about lambda.
+// The accumulators are flushed on row change (complete rows to the Gram
+// table, chunk-straddling runs to the partial tables for the fix-up).
 #include "gt_contract.hpp"
 
 namespace gt {
@@ -34,8 +45,10 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 enum DeltaKind { kValu = 0, kBf16 = 1, kBf16R = 2 };
+constexpr int kTP = 17; // LDS pitch of a delta tile row (odd: fewer bank conflicts)
 
 __device__ inline short bf16_bits(float x) { // round to nearest even
   uint32_t u = __float_as_uint(x);
@@ -46,43 +59,53 @@ __device__ inline float bf16_value(short h) {
   return __uint_as_float(((uint32_t)(uint16_t)h) << 16);
 }
 
-constexpr int kTP = 17; // LDS pitch of a delta tile row (odd: fewer bank conflicts)
-
-template <int R>
-__device__ inline int packed_index(int p, int q) {
+template <int R> __device__ inline int packed_index(int p, int q) {
   if (p <= q && q < R) return p * R - p * (p - 1) / 2 + (q - p);
   if (p < R && q == R) return R * (R + 1) / 2 + p;
   return -1;
+}
+
+template <int KIND> __host__ __device__ constexpr size_t lds_floats_per_wave(int slots, int R,
+                                                                             int O) {
+  return (slots >= 2 ? (size_t)slots * R * kWave : 0) + (KIND == kValu ? 0 : (size_t)O * kWave);
 }
 
 template <int R, int KIND>
 __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
   static_assert(R <= 15, "MFMA Gram covers R <= 15");
   constexpr int W = gram_width(R);
+  constexpr int RS = row_stride(R);
   __shared__ float s_tile[4][kWave * kTP];
   __shared__ uint32_t s_row[4][kWave];
   __shared__ uint32_t s_iL[4][kWave];
   __shared__ float s_x[4][kWave];
   extern __shared__ float dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int slots = a.slots;
+  // the core operand table (BF16 kinds) goes to LDS once per workgroup
+  s4 *s_gb = reinterpret_cast<s4 *>(dyn + (size_t)wpb * lds_floats_per_wave<KIND>(slots, R, O));
+  if constexpr (KIND != kValu) {
+    const int tab = (KIND == kBf16R ? 2 : 1) * O * kWave;
+    const s4 *src = reinterpret_cast<const s4 *>(a.Gb);
+    for (int i = threadIdx.x; i < tab; i += blockDim.x) s_gb[i] = src[i];
+    __syncthreads();
+  }
   const uint64_t c = (uint64_t)blockIdx.x * wpb + w;
   if (c >= a.nchunks) return;
-  const int N = a.order, n = a.mode;
-  const int L = (n == N - 1) ? N - 2 : N - 1;
-  const int slots = (N > 2 ? N - 2 : 0);
-  const size_t per_wave = (size_t)slots * R * kWave + (KIND == kValu ? 0 : (size_t)O * kWave);
-  float *lrows = dyn + (size_t)w * per_wave;
-  float *souter = lrows + (size_t)slots * R * kWave;
+  float *lrows = dyn + (size_t)w * lds_floats_per_wave<KIND>(slots, R, O);
+  float *souter = lrows + (slots >= 2 ? (size_t)slots * R * kWave : 0);
   float *tile = s_tile[w];
   const uint64_t cb = c * a.chunk;
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
   const int q = lane & 15, g = lane >> 4;
 
+  // v_mfma_f64_16x16x4f64 leaves D[p][q] in lane 16 (p % 4) + q, register
+  // p / 4 (rows interleaved across the lane groups)
   int pk[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) pk[r] = packed_index<R>(4 * g + r, q);
-  double acc64[4] = {0.0, 0.0, 0.0, 0.0};
-  uint32_t cur = __builtin_amdgcn_readfirstlane(a.st.idx[n][cb]);
+  for (int r = 0; r < 4; ++r) pk[r] = packed_index<R>(g + 4 * r, q);
+  d4 acc64 = {0.0, 0.0, 0.0, 0.0};
+  uint32_t cur = __builtin_amdgcn_readfirstlane(a.idx_row[cb]);
   uint64_t run_begin = cb;
 
   auto flush = [&](uint32_t row, uint64_t b, uint64_t e_end) {
@@ -102,47 +125,62 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
   for (uint64_t tb = cb; tb < ce; tb += kWave) {
     const uint64_t e = tb + lane;
     const bool valid = e < ce;
+    const uint64_t ee = valid ? e : cb; // in-bounds position for padding lanes
     const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
-    uint32_t gi[kMaxOrder];
+    // ---- coalesced stream reads, then every gather of the tile
+    const uint32_t row = a.idx_row[ee];
+    const uint32_t iL = a.idx_L[ee];
+    uint32_t io[kSlots];
 #pragma unroll
-    for (int k = 0; k < kMaxOrder; ++k) gi[k] = (k < N && valid) ? a.st.idx[k][e] : 0u;
-    const float x = valid ? a.st.val[e] : 0.f;
-    s_row[w][lane] = gi[n];
+    for (int t = 0; t < kSlots; ++t) io[t] = (t < slots) ? a.idx_O[t][ee] : 0u;
+    const float x = valid ? a.st.val[ee] : 0.f;
+    s_row[w][lane] = valid ? row : 0xFFFFFFFFu;
     float v[4][4];
     if constexpr (KIND == kValu) {
       float d[R];
-      contract<R>(N, n, a.A, a.Gt, gi, lrows, lane, d);
+      contract_slots<R>(slots, a.A_L, iL, a.A_O, io, a.Gt, lrows, lane, d);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) tile[lane * kTP + r] = (r < R) ? (valid ? d[r] : 0.f) : (r == R ? x : 0.f);
+      for (int r = 0; r < 16; ++r)
+        tile[lane * kTP + r] = (r < R) ? (valid ? d[r] : 0.f) : (r == R ? x : 0.f);
       wave_sync();
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[mb][i] = tile[(mb * 16 + 4 * g + i) * kTP + q];
     } else {
-      // outer rows of this lane's nonzero, then s_m[o] for every o
+      // outer rows of this lane's nonzero
+      float aO[RS];
+      if (slots == 1) load_row<R>(a.A_O[0], io[0], aO);
+      if (slots >= 2) {
 #pragma unroll
-      for (int k = 0; k < kMaxOrder; ++k) {
-        if (k >= N || k == n || k == L) continue;
-        const float *row = a.A[k] + (size_t)gi[k] * R;
-        const int t = k - (k > n ? 1 : 0);
+        for (int t = 0; t < kSlots; ++t) {
+          if (t >= slots) continue;
+          float rt[RS];
+          load_row<R>(a.A_O[t], io[t], rt);
 #pragma unroll
-        for (int j = 0; j < R; ++j) lrows[(t * R + j) * kWave + lane] = row[j];
+          for (int j = 0; j < R; ++j) lrows[(t * R + j) * kWave + lane] = rt[j];
+        }
       }
-      s_iL[w][lane] = gi[L];
+      // s_m[o] for this lane's nonzero, into LDS for the stage-2 layout
+      s_iL[w][lane] = iL;
       s_x[w][lane] = x;
-      {
-        int dig[kMaxOrder - 2];
+      if (slots == 0) {
+        souter[lane] = valid ? 1.f : 0.f;
+      } else if (slots == 1) {
 #pragma unroll
-        for (int t = 0; t < kMaxOrder - 2; ++t) dig[t] = 0;
+        for (int o = 0; o < R; ++o) souter[o * kWave + lane] = valid ? aO[o] : 0.f;
+      } else {
+        int dig[kSlots];
+#pragma unroll
+        for (int t = 0; t < kSlots; ++t) dig[t] = 0;
         for (int o = 0; o < O; ++o) {
           float s = valid ? 1.f : 0.f;
 #pragma unroll
-          for (int t = 0; t < kMaxOrder - 2; ++t)
+          for (int t = 0; t < kSlots; ++t)
             if (t < slots) s *= lrows[(t * R + dig[t]) * kWave + lane];
           souter[o * kWave + lane] = s;
 #pragma unroll
-          for (int t = kMaxOrder - 3; t >= 0; --t) {
+          for (int t = kSlots - 1; t >= 0; --t) {
             if (t < slots) {
               if (++dig[t] < R) break;
               dig[t] = 0;
@@ -155,14 +193,14 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
       s4 ahi[4], alo[4];
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
-        const float *rowL = a.A[L] + (size_t)s_iL[w][mb * 16 + q] * R;
+        f4 val = {0.f, 0.f, 0.f, 0.f};
+        if (4 * g < RS)
+          val = *reinterpret_cast<const f4 *>(a.A_L + (size_t)s_iL[w][mb * 16 + q] * RS + 4 * g);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          const int k = 4 * g + t;
-          const float val = (k < R) ? rowL[k] : 0.f;
-          const short h = bf16_bits(val);
+          const short h = bf16_bits(val[t]);
           ahi[mb][t] = h;
-          alo[mb][t] = bf16_bits(val - bf16_value(h));
+          alo[mb][t] = bf16_bits(val[t] - bf16_value(h));
         }
       }
       float dacc[4][4];
@@ -170,12 +208,10 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
       for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
         for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
-      const s4 *__restrict__ gb_hi = reinterpret_cast<const s4 *>(a.Gb);
-      const s4 *__restrict__ gb_lo = gb_hi + (size_t)O * kWave;
-      for (int o = 0; o < O; ++o) {
-        const s4 bh = gb_hi[(size_t)o * kWave + lane];
+      auto stage = [&](int o) {
+        const s4 bh = s_gb[o * kWave + lane];
         s4 bl;
-        if constexpr (KIND == kBf16R) bl = gb_lo[(size_t)o * kWave + lane];
+        if constexpr (KIND == kBf16R) bl = s_gb[(O + o) * kWave + lane];
         f4 u[4];
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
@@ -196,6 +232,14 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv[i], dacc[mb][i]);
         }
+      };
+      if (slots == 0) {
+        stage(0);
+      } else if (slots == 1) { // N = 3: O = R, unrolled so the operand reads pipeline
+#pragma unroll
+        for (int o = 0; o < R; ++o) stage(o);
+      } else {
+        for (int o = 0; o < O; ++o) stage(o);
       }
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
@@ -219,7 +263,6 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
       }
       const uint64_t above = (s >= 63) ? 0ull : (starts & (~0ull << (s + 1)));
       const int e_run = above ? (int)__builtin_ctzll(above) : cnt;
-      f4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
         const int lo = s > mb * 16 ? s : mb * 16;
@@ -229,12 +272,10 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
         for (int i = 0; i < 4; ++i) {
           if (lo + ((i - lo) & 3) >= hi) continue; // no nonzero of this (mb, i) in the run
           const int m = mb * 16 + 4 * g + i;
-          const float val = (m >= s && m < e_run) ? v[mb][i] : 0.f;
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(val, val, acc, 0, 0, 0);
+          const double val = (m >= s && m < e_run) ? (double)v[mb][i] : 0.0;
+          acc64 = __builtin_amdgcn_mfma_f64_16x16x4f64(val, val, acc64, 0, 0, 0);
         }
       }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc64[r] += (double)acc[r];
       s = e_run;
     }
     wave_sync();
@@ -244,14 +285,13 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
 
 template <int R, int KIND> void launch_impl(const GramArgs &a, int O, hipStream_t st) {
   if (a.nchunks == 0) return;
-  const int slots = a.order > 2 ? a.order - 2 : 0;
-  const size_t per_wave =
-      ((size_t)slots * R * kWave + (KIND == kValu ? 0 : (size_t)O * kWave)) * sizeof(float);
+  const size_t per_wave = lds_floats_per_wave<KIND>(a.slots, R, O) * sizeof(float);
+  const size_t table = KIND == kValu ? 0 : (size_t)(KIND == kBf16R ? 2 : 1) * O * kWave * 8;
   int wpb = 4;
-  while (wpb > 1 && per_wave * wpb > 64 * 1024) --wpb;
+  while (wpb > 1 && per_wave * wpb + table > 64 * 1024) --wpb;
   const unsigned grid = (unsigned)((a.nchunks + wpb - 1) / wpb);
-  hipLaunchKernelGGL((k_gram_mfma<R, KIND>), dim3(grid), dim3(64 * wpb), per_wave * wpb, st, a,
-                     wpb, O);
+  hipLaunchKernelGGL((k_gram_mfma<R, KIND>), dim3(grid), dim3(64 * wpb), per_wave * wpb + table,
+                     st, a, wpb, O);
 }
 
 template <int KIND> void dispatch(int R, const GramArgs &a, int O, hipStream_t st) {

@@ -22,6 +22,10 @@ constexpr int kTile = 64;      // nonzeros per tile: one per lane
 // followed by C (R doubles).
 __host__ __device__ constexpr int gram_width(int R) { return R * (R + 1) / 2 + R; }
 
+// Factor rows are stored with a stride of R rounded up to 4 floats (padding
+// zero) so a row is read with 16-byte loads.
+__host__ __device__ constexpr int row_stride(int R) { return (R + 3) & ~3; }
+
 // Ranks the templated kernels are instantiated for.  A rank outside this list
 // is rejected with GT_ERR_UNSUPPORTED by the C ABI.
 #define GT_FOR_EACH_RANK(X) \
@@ -72,8 +76,11 @@ struct PartArgs {
   uint32_t part[kMaxOrder];
 };
 // keys[e] = block id of nonzero e (tensor_manager.tpp:232-242), vals[e] = e.
-void block_ids(const PartArgs &a, uint32_t *keys, uint32_t *vals, uint32_t *block_counts,
-               hipStream_t s);
+void block_ids(const PartArgs &a, uint32_t blocks, uint32_t *keys, uint32_t *vals,
+               uint32_t *block_counts, hipStream_t s);
+// Row pointer (rows+1 entries) of n keys sorted ascending, all < rows.
+void row_ptr_from_sorted(const uint32_t *keys, uint64_t n, uint64_t rows, uint64_t *row_ptr,
+                         hipStream_t s);
 struct U32Ptrs {
   const uint32_t *p[kMaxOrder];
 };
@@ -119,6 +126,14 @@ struct GramArgs {
   uint64_t chunk;             // nonzeros per chunk (multiple of 64)
   uint64_t nchunks;
   const void *Gb;             // BF16 core operand table (hi, then lo): [o][lane][4]
+  // the same operands in slot order, so the kernel never indexes by mode:
+  // L = last mode other than n, outer t = the other modes ascending
+  int slots;                             // N - 2
+  const uint32_t *idx_row;               // st.idx[n]
+  const uint32_t *idx_L;                 // st.idx[L]
+  const uint32_t *idx_O[kMaxOrder - 2];  // st.idx[outer t]
+  const float *A_L;
+  const float *A_O[kMaxOrder - 2];
 };
 void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s);
 void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);
@@ -136,19 +151,23 @@ struct SolveArgs {
 };
 void launch_solve(int R, const SolveArgs &a, hipStream_t s);
 
-// Contraction over block-ordered COO: delta of `mode` (write_delta), xhat, and
-// per-workgroup FP64 partial sums of (x - xhat)^2.
+// Contraction over block-ordered COO for one mode n, operands in slot order
+// (as GramArgs): delta of every nonzero (delta_out), xhat = A_n[i_n] . delta
+// (xhat_out) and per-workgroup FP64 partial sums of (x - xhat)^2.
 struct CooArgs {
-  int order;
-  int mode;                        // delta mode; for xhat this is order-1
+  int slots;                             // N - 2
   uint64_t nnz;
-  const uint32_t *idx[kMaxOrder];  // global indices, block order
+  const uint32_t *idx_n;                 // global indices, block order
+  const uint32_t *idx_L;
+  const uint32_t *idx_O[kMaxOrder - 2];
   const float *val;
-  const float *A[kMaxOrder];
-  const float *Gt;                 // unfolded core of `mode`
-  float *delta_out;                // nnz x R or null
-  float *xhat_out;                 // nnz or null
-  double *err_partials;            // gridDim.x or null
+  const float *A_n;
+  const float *A_L;
+  const float *A_O[kMaxOrder - 2];
+  const float *Gt;                       // unfolded core of mode n
+  float *delta_out;                      // nnz x R or null
+  float *xhat_out;                       // nnz or null
+  double *err_partials;                  // gridDim.x or null
 };
 int launch_coo_contract(int R, const CooArgs &a, hipStream_t s); // returns grid size
 void reduce_f64(const double *in, int n, double *out, hipStream_t s);

@@ -13,9 +13,18 @@ unsigned grid_for(uint64_t n, unsigned threads, unsigned cap = 1u << 20) {
   return (unsigned)g;
 }
 
-// block id = sum_n floor(i_n / bd_n) * prod_{m>n} P_m (tensor_manager.tpp:232-242)
-__global__ void k_block_ids(PartArgs a, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals,
-                            uint32_t *__restrict__ counts) {
+constexpr uint32_t kLdsBins = 4096;
+
+// block id = sum_n floor(i_n / bd_n) * prod_{m>n} P_m (tensor_manager.tpp:232-242).
+// Counts go to a per-workgroup LDS histogram first (skewed tensors put most
+// nonzeros in one block, and one global counter would serialise them).
+__global__ void k_block_ids(PartArgs a, uint32_t blocks, uint32_t *__restrict__ keys,
+                            uint32_t *__restrict__ vals, uint32_t *__restrict__ counts) {
+  __shared__ uint32_t h[kLdsBins];
+  const bool lds = blocks <= kLdsBins;
+  if (lds)
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) h[b] = 0;
+  __syncthreads();
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nnz;
        e += (uint64_t)gridDim.x * blockDim.x) {
     uint32_t id = 0;
@@ -24,7 +33,26 @@ __global__ void k_block_ids(PartArgs a, uint32_t *__restrict__ keys, uint32_t *_
       if (k < a.order) id = id * a.part[k] + a.idx[k][e] / a.bdims[k];
     keys[e] = id;
     vals[e] = (uint32_t)e;
-    atomicAdd(&counts[id], 1u);
+    if (lds)
+      atomicAdd(&h[id], 1u);
+    else
+      atomicAdd(&counts[id], 1u);
+  }
+  __syncthreads();
+  if (lds)
+    for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x)
+      if (h[b]) atomicAdd(&counts[b], h[b]);
+}
+
+// row_ptr[r] = first position whose sorted key is >= r, for r in [0, rows];
+// every row pointer is written by exactly one thread.
+__global__ void k_row_ptr_from_sorted(const uint32_t *__restrict__ keys, uint64_t n,
+                                      uint64_t rows, uint64_t *__restrict__ row_ptr) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e <= n;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t lo = (e == 0) ? 0 : (uint64_t)keys[e - 1] + 1;
+    const uint64_t hi = (e == n) ? rows : (uint64_t)keys[e];
+    for (uint64_t r = lo; r <= hi; ++r) row_ptr[r] = e;
   }
 }
 
@@ -92,11 +120,18 @@ __global__ void k_sub_u32(const uint32_t *__restrict__ idx, uint32_t base, uint6
 
 } // namespace
 
-void block_ids(const PartArgs &a, uint32_t *keys, uint32_t *vals, uint32_t *block_counts,
-               hipStream_t s) {
+void block_ids(const PartArgs &a, uint32_t blocks, uint32_t *keys, uint32_t *vals,
+               uint32_t *block_counts, hipStream_t s) {
   if (!a.nnz) return;
-  hipLaunchKernelGGL(k_block_ids, dim3(grid_for(a.nnz, 256)), dim3(256), 0, s, a, keys, vals,
-                     block_counts);
+  hipLaunchKernelGGL(k_block_ids, dim3(grid_for(a.nnz, 256, 4096)), dim3(256), 0, s, a, blocks,
+                     keys, vals, block_counts);
+  GT_HIP(hipGetLastError());
+}
+
+void row_ptr_from_sorted(const uint32_t *keys, uint64_t n, uint64_t rows, uint64_t *row_ptr,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_row_ptr_from_sorted, dim3(grid_for(n + 1, 256)), dim3(256), 0, s, keys, n,
+                     rows, row_ptr);
   GT_HIP(hipGetLastError());
 }
 

@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(64) k_solve_lane(SolveArgs a) {
   constexpr int W = gram_width(R);
   const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= a.rows) return;
-  float *out = a.A + row * R;
+  float *out = a.A + row * row_stride(R);
   if (a.row_ptr[row + 1] == a.row_ptr[row]) {
 #pragma unroll
     for (int r = 0; r < R; ++r) out[r] = 0.f;
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(64) k_solve_wave(SolveArgs a) {
   __shared__ double y[R];
   const uint64_t row = blockIdx.x;
   const int lane = threadIdx.x;
-  float *out = a.A + row * R;
+  float *out = a.A + row * row_stride(R);
   if (a.row_ptr[row + 1] == a.row_ptr[row]) {
     if (lane < R) out[lane] = 0.f;
     return;

@@ -195,3 +195,24 @@ def test_fit_vs_fp64_oracle_by_precision(prec, tol):
     f64, _ = oracle.als(t, A, G, 1e-3, 5, 0.0, "f64")
     rel = np.abs(fits - f64) / np.abs(f64)
     assert rel.max() < tol, (prec, fits, f64)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16r", 1e-4)])
+def test_skewed_sample_of_benchmark_workload(prec, tol):
+    """2e6 nonzeros drawn like the benchmark workload (1e6^3 index space,
+    Zipf(1.1) per mode, default lambda = 0.0001f, P = 4x4x4): most rows have
+    fewer than R nonzeros and a few have ~1e5, so the row systems range from
+    lambda-dominated to heavily loaded.  One ALS iteration vs the FP64 oracle."""
+    from gputucker_amd import synth
+    idx, vals, dims = synth.generate([1_000_000] * 3, 2_000_000, [("zipf", 1.1)] * 3,
+                                     "uniform", seed=3)
+    A, G = synth.init_factors(dims, 10, seed=3)
+    lam = gt.default_lambda()
+    e = make(idx, vals, dims, 10, [4, 4, 4], precision=prec, max_iters=1, tol=0.0)
+    e.set_init(A, G)
+    fits = e.run()
+    t = oracle.Tensor(idx, vals.astype(np.float64), dims, [4, 4, 4], 10)
+    Ap = [oracle.pad_factor(a.astype(np.float64), t.rows(k)) for k, a in enumerate(A)]
+    f64, _ = oracle.als(t, Ap, G.astype(np.float64), lam, 1, 0.0, "f64")
+    assert np.isfinite(fits).all()
+    assert abs(fits[0] - f64[0]) / abs(f64[0]) < tol, (fits, f64)
