@@ -64,6 +64,8 @@ struct GtError : std::runtime_error {
   GtError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
 
+constexpr uint64_t kErrPartialsPerSegment = 2048; // the COO contraction's grid cap
+
 int ceil_log2(uint64_t v) { // bits needed to hold values < v
   int b = 0;
   while (b < 64 && (1ull << b) < v) ++b;
@@ -189,7 +191,19 @@ struct gt_ctx {
   int O = 1;                         // R^(N-2)
   // work
   DevBuf<double> gram, bpart0, bpart1, err_part, scalar;
+  DevBuf<uint32_t> chunk_tail; // per chunk: row continuing into the next chunk, or ~0u
+  DevBuf<double> fit_part;     // residual partials of the row solves
   uint64_t chunk = 0, nchunks = 0;
+  // out-of-core pipeline: the per-mode streams live in pinned host memory and
+  // are pumped through two device staging buffers on a copy stream
+  bool streaming = false;
+  uint64_t seg_nnz = 0;
+  uint32_t *h_idx[GT_MAX_ORDER][GT_MAX_ORDER] = {};
+  float *h_val[GT_MAX_ORDER] = {};
+  DevBuf<uint32_t> stage_idx[2][GT_MAX_ORDER];
+  DevBuf<float> stage_val[2];
+  hipStream_t s_copy = nullptr;
+  hipEvent_t ev_copy[2] = {}, ev_comp[2] = {};
   // timing
   hipEvent_t ev[8] = {};
   gt_timings tm{};
@@ -205,6 +219,20 @@ struct gt_ctx {
     hipEvent_t e;
     GT_HIP(hipEventCreate(&e));
     return e;
+  }
+  void release_host_streams() {
+    for (int n = 0; n < GT_MAX_ORDER; ++n) {
+      for (int k = 0; k < GT_MAX_ORDER; ++k)
+        if (h_idx[n][k]) {
+          (void)hipHostFree(h_idx[n][k]);
+          h_idx[n][k] = nullptr;
+        }
+      if (h_val[n]) {
+        (void)hipHostFree(h_val[n]);
+        h_val[n] = nullptr;
+      }
+    }
+    streaming = false;
   }
   void drain_profile() {
     for (auto &pr : prof_pending) {
@@ -228,6 +256,15 @@ struct gt_ctx {
       for (auto e : ev_pool) (void)hipEventDestroy(e);
       for (auto e : ev)
         if (e) (void)hipEventDestroy(e);
+      release_host_streams();
+      if (s_copy) {
+        (void)hipStreamSynchronize(s_copy);
+        (void)hipStreamDestroy(s_copy);
+      }
+      for (int b = 0; b < 2; ++b) {
+        if (ev_copy[b]) (void)hipEventDestroy(ev_copy[b]);
+        if (ev_comp[b]) (void)hipEventDestroy(ev_comp[b]);
+      }
       (void)hipStreamDestroy(s);
     }
   }
@@ -304,20 +341,46 @@ float from_bf16(uint16_t h) {
 // order: lane l, element t holds G[o][jL = 4(l/16)+t][r = l%16] (zero outside
 // R x R).  The hi table (BF16 of G) is followed by the lo table (BF16 of the
 // rounding remainder G - hi).
+// BF16 core operand tables of the MFMA delta (layout and plans in
+// gt_internal.hpp).  Entry (o, lane, t) of a table is the B operand word of
+// K slot t for output column r = lane % 16 of outer index o; G[o][jL][r] is
+// zero outside jL, r < R.
 std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R) {
   const int O = (int)ipow(R, N > 2 ? N - 2 : 0);
-  std::vector<uint16_t> tab((size_t)2 * O * kWave * 4, 0);
+  const size_t n16 = gb_k16_words(O), n32 = gb_k32_words(O);
+  std::vector<uint16_t> tab(2 * n16 + n32, 0);
+  auto g_at = [&](int o, int jL, int r) -> float {
+    return (jL < R && r < R) ? gt[((size_t)o * R + jL) * R + r] : 0.f;
+  };
+  auto hi = [&](float g) { return to_bf16(g); };
+  auto lo = [&](float g) { return to_bf16(g - from_bf16(to_bf16(g))); };
   for (int o = 0; o < O; ++o)
-    for (int l = 0; l < kWave; ++l)
+    for (int l = 0; l < kWave; ++l) {
+      const int g = l / 16, r = l % 16;
       for (int t = 0; t < 4; ++t) {
-        const int jL = 4 * (l / 16) + t, r = l % 16;
-        float g = 0.f;
-        if (jL < R && r < R) g = gt[((size_t)o * R + jL) * R + r];
-        const uint16_t hi = to_bf16(g);
         const size_t at = ((size_t)o * kWave + l) * 4 + t;
-        tab[at] = hi;
-        tab[(size_t)O * kWave * 4 + at] = to_bf16(g - from_bf16(hi));
+        tab[at] = hi(g_at(o, 4 * g + t, r));                 // K16 hi
+        tab[n16 + n32 + at] = lo(g_at(o, 4 * g + t, r));     // K16 lo
       }
+      uint16_t *k32 = tab.data() + n16 + ((size_t)o * kWave + l) * 8;
+      switch (bf16r_plan(R, g)) {
+      case kPlanHL:
+        for (int t = 0; t < 8; ++t) k32[t] = hi(g_at(o, 4 * g + (t & 3), r));
+        break;
+      case kPlanHH:
+        for (int t = 0; t < 8; ++t) k32[t] = lo(g_at(o, t, r));
+        break;
+      case kPlanR10:
+        for (int t = 0; t < 4; ++t) k32[t] = hi(g_at(o, 8 + t, r));
+        k32[4] = hi(g_at(o, 8, r));
+        k32[5] = hi(g_at(o, 9, r));
+        k32[6] = lo(g_at(o, 8, r));
+        k32[7] = lo(g_at(o, 9, r));
+        break;
+      default:
+        break;
+      }
+    }
   return tab;
 }
 
@@ -336,8 +399,63 @@ void ensure_work(gt_ctx *c) {
   c->nchunks = (c->nnz + c->chunk - 1) / c->chunk;
   c->bpart0.alloc(std::max<uint64_t>(c->nchunks, 1) * W);
   c->bpart1.alloc(std::max<uint64_t>(c->nchunks, 1) * W);
+  c->chunk_tail.alloc(std::max<uint64_t>(c->nchunks, 1));
+  uint64_t parts = 1;
+  for (int n = 0; n < c->N; ++n) parts = std::max(parts, solve_partials(c->R, c->rows[n]));
+  c->fit_part.alloc(parts);
   c->err_part.alloc(4096);
   c->scalar.alloc(16);
+}
+
+// With a device budget smaller than the resident layout, the per-mode streams
+// move to pinned host memory after partitioning and each mode update pumps
+// them through two staging buffers (the reference's async block pipeline,
+// delta.cuh:165-213, with the tasks of scheduler.tpp:85-159 replaced by
+// fixed-size segments of the row-ordered stream).  The partition itself runs
+// with the tensor resident.
+void maybe_stream(gt_ctx *c) {
+  const uint64_t budget = c->o.device_budget;
+  if (budget == 0) return;
+  const int N = c->N, R = c->R, W = gram_width(R);
+  const uint64_t nnz = c->nnz;
+  const uint64_t movable = nnz * 4ull * (uint64_t)(N * (N + 1) + (N + 2) + N);
+  uint64_t fixed = (c->gram.n + c->bpart0.n + c->bpart1.n) * sizeof(double) + c->chunk_tail.n * 4;
+  for (int n = 0; n < N; ++n)
+    fixed += c->rows[n] * (uint64_t)row_stride(R) * 4 + (c->rows[n] + 1) * 8;
+  fixed += (uint64_t)N * ipow(R, N) * 4 * 2 + (1u << 20);
+  if (movable + fixed <= budget) return; // fits: stay resident
+  if (fixed >= budget) throw GtError(GT_ERR_MEMORY, "device budget is below the per-row state");
+  uint64_t seg = (budget - fixed) / (2ull * (N + 1) * 4);
+  if (c->plan.avail) seg = std::min(seg, c->plan.avail);
+  seg = (seg / c->chunk) * c->chunk;
+  if (seg == 0) throw GtError(GT_ERR_MEMORY, "device budget too small for one staged chunk");
+  (void)W;
+  for (int n = 0; n < N; ++n) {
+    for (int k = 0; k < N; ++k) {
+      GT_HIP(hipHostMalloc(&c->h_idx[n][k], nnz * sizeof(uint32_t), hipHostMallocDefault));
+      GT_HIP(hipMemcpy(c->h_idx[n][k], c->st_idx[n][k].p, nnz * sizeof(uint32_t),
+                       hipMemcpyDeviceToHost));
+      c->st_idx[n][k].reset();
+    }
+    GT_HIP(hipHostMalloc(&c->h_val[n], nnz * sizeof(float), hipHostMallocDefault));
+    GT_HIP(hipMemcpy(c->h_val[n], c->st_val[n].p, nnz * sizeof(float), hipMemcpyDeviceToHost));
+    c->st_val[n].reset();
+    c->where[n].reset();
+  }
+  for (int k = 0; k < N; ++k) c->coo_idx[k].reset();
+  c->coo_val.reset();
+  c->coo_blk.reset();
+  for (int b = 0; b < 2; ++b) {
+    for (int k = 0; k < N; ++k) c->stage_idx[b][k].alloc(seg);
+    c->stage_val[b].alloc(seg);
+    if (!c->ev_copy[b]) GT_HIP(hipEventCreateWithFlags(&c->ev_copy[b], hipEventDisableTiming));
+    if (!c->ev_comp[b]) GT_HIP(hipEventCreateWithFlags(&c->ev_comp[b], hipEventDisableTiming));
+  }
+  if (!c->s_copy) GT_HIP(hipStreamCreateWithFlags(&c->s_copy, hipStreamNonBlocking));
+  const uint64_t nseg = (nnz + seg - 1) / seg;
+  c->err_part.alloc(std::max<uint64_t>(4096, nseg * kErrPartialsPerSegment));
+  c->seg_nnz = seg;
+  c->streaming = true;
 }
 
 GramArgs gram_args(gt_ctx *c, int n) {
@@ -355,8 +473,12 @@ GramArgs gram_args(gt_ctx *c, int n) {
   a.gram = c->gram.p;
   a.bpart0 = c->bpart0.p;
   a.bpart1 = c->bpart1.p;
+  a.chunk_tail = c->chunk_tail.p;
   a.chunk = c->chunk;
   a.nchunks = c->nchunks;
+  a.chunk_begin = 0;
+  a.chunk_end = c->nchunks;
+  a.rows = c->rows[n];
   a.Gb = c->Gb[n].p;
   // slot order: L = last mode other than n, outer slots = the rest ascending
   const int L = (n == c->N - 1) ? c->N - 2 : c->N - 1;
@@ -374,8 +496,21 @@ GramArgs gram_args(gt_ctx *c, int n) {
   return a;
 }
 
-void run_gram(gt_ctx *c, int n) {
-  GramArgs a = gram_args(c, n);
+// Points the stream operands of `a` (mode n) at idx[k] / val, indexed by
+// global stream position.
+void set_stream_ptrs(GramArgs &a, const gt_ctx *c, int n, uint32_t *const *idx,
+                     const float *val) {
+  const int L = (n == c->N - 1) ? c->N - 2 : c->N - 1;
+  for (int k = 0; k < c->N; ++k) a.st.idx[k] = idx[k];
+  a.st.val = val;
+  a.idx_row = idx[n];
+  a.idx_L = idx[L];
+  int t = 0;
+  for (int k = 0; k < c->N; ++k)
+    if (k != n && k != L) a.idx_O[t++] = idx[k];
+}
+
+void launch_gram_window(gt_ctx *c, const GramArgs &a) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->o.profile) {
     e0 = c->get_event();
@@ -390,10 +525,49 @@ void run_gram(gt_ctx *c, int n) {
     GT_HIP(hipEventRecord(e1, c->s));
     c->prof_pending.emplace_back(e0, e1);
   }
+}
+
+// Out-of-core: copies segment k of mode n's stream into staging buffer k % 2
+// on the copy stream; the compute stream waits for it.  Returns [lo, hi).
+std::pair<uint64_t, uint64_t> stage_segment(gt_ctx *c, int n, uint64_t k) {
+  const int b = (int)(k & 1);
+  const uint64_t lo = k * c->seg_nnz, hi = std::min(lo + c->seg_nnz, c->nnz), len = hi - lo;
+  if (k >= 2) GT_HIP(hipStreamWaitEvent(c->s_copy, c->ev_comp[b], 0)); // buffer free again
+  for (int m = 0; m < c->N; ++m)
+    GT_HIP(hipMemcpyAsync(c->stage_idx[b][m].p, c->h_idx[n][m] + lo, len * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, c->s_copy));
+  GT_HIP(hipMemcpyAsync(c->stage_val[b].p, c->h_val[n] + lo, len * sizeof(float),
+                        hipMemcpyHostToDevice, c->s_copy));
+  GT_HIP(hipEventRecord(c->ev_copy[b], c->s_copy));
+  GT_HIP(hipStreamWaitEvent(c->s, c->ev_copy[b], 0));
+  return {lo, hi};
+}
+
+void run_gram(gt_ctx *c, int n) {
+  GramArgs a = gram_args(c, n);
+  if (!c->streaming) {
+    launch_gram_window(c, a);
+  } else {
+    const uint64_t nseg = (c->nnz + c->seg_nnz - 1) / c->seg_nnz;
+    for (uint64_t k = 0; k < nseg; ++k) {
+      const int b = (int)(k & 1);
+      const auto [lo, hi] = stage_segment(c, n, k);
+      uint32_t *ptrs[GT_MAX_ORDER];
+      for (int m = 0; m < c->N; ++m) ptrs[m] = c->stage_idx[b][m].p - lo; // global positions
+      set_stream_ptrs(a, c, n, ptrs, c->stage_val[b].p - lo);
+      a.chunk_begin = lo / c->chunk;
+      a.chunk_end = (hi + c->chunk - 1) / c->chunk;
+      launch_gram_window(c, a);
+      GT_HIP(hipEventRecord(c->ev_comp[b], c->s));
+    }
+  }
   launch_gram_fixup(c->R, a, c->s);
 }
 
-void mode_update(gt_ctx *c, int n) {
+// One mode update.  With fit_partials, the row solves also leave the
+// residual partials of the updated factor (valid as a fit only for the last
+// mode of an iteration); returns how many were written.
+uint64_t mode_update(gt_ctx *c, int n, bool fit_partials = false) {
   require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
   require(n >= 0 && n < c->N, GT_ERR_ARG, "mode out of range");
   if (c->prof_pending.size() > 64) c->drain_profile();
@@ -406,7 +580,8 @@ void mode_update(gt_ctx *c, int n) {
   sa.rows = c->rows[n];
   sa.lambda = c->o.lambda;
   sa.A = c->A[n].p;
-  launch_solve(c->R, sa, c->s);
+  sa.fit_partials = fit_partials ? c->fit_part.p : nullptr;
+  const uint64_t parts = launch_solve(c->R, sa, c->s);
   GT_HIP(hipEventRecord(c->ev[2], c->s));
   GT_HIP(hipEventSynchronize(c->ev[2]));
   float g = 0.f, s = 0.f;
@@ -415,6 +590,7 @@ void mode_update(gt_ctx *c, int n) {
   c->tm.ms_gram[n] = g;
   c->tm.ms_solve[n] = s;
   c->tm.ms_mode[n] = g + s;
+  return parts;
 }
 
 // I_n x R host rows into the padded device layout (one contiguous copy).
@@ -451,10 +627,37 @@ CooArgs coo_args(gt_ctx *c, int mode) {
 double compute_fit(gt_ctx *c) {
   require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
   GT_HIP(hipEventRecord(c->ev[3], c->s));
-  CooArgs a = coo_args(c, c->N - 1);
-  a.err_partials = c->err_part.p;
-  int g = launch_coo_contract(c->R, a, c->s);
-  reduce_f64(c->err_part.p, g, c->scalar.p, c->s);
+  if (!c->streaming) {
+    CooArgs a = coo_args(c, c->N - 1);
+    a.err_partials = c->err_part.p;
+    int g = launch_coo_contract(c->R, a, c->s);
+    reduce_f64(c->err_part.p, g, c->scalar.p, c->s);
+  } else { // any order of the nonzeros gives the same sum: stream mode N-1's copy
+    const int n = c->N - 1, L = c->N - 2;
+    const uint64_t nseg = (c->nnz + c->seg_nnz - 1) / c->seg_nnz;
+    fill_f64(c->err_part.p, 0.0, nseg * kErrPartialsPerSegment, c->s);
+    for (uint64_t k = 0; k < nseg; ++k) {
+      const int b = (int)(k & 1);
+      const auto [lo, hi] = stage_segment(c, n, k);
+      CooArgs a{};
+      a.slots = c->N - 2;
+      a.nnz = hi - lo;
+      a.idx_n = c->stage_idx[b][n].p;
+      a.idx_L = c->stage_idx[b][L].p;
+      a.A_n = c->A[n].p;
+      a.A_L = c->A[L].p;
+      for (int t = 0; t < c->N - 2; ++t) {
+        a.idx_O[t] = c->stage_idx[b][t].p; // outer slots of mode N-1: modes 0..N-3
+        a.A_O[t] = c->A[t].p;
+      }
+      a.val = c->stage_val[b].p;
+      a.Gt = c->Gt[n].p;
+      a.err_partials = c->err_part.p + k * kErrPartialsPerSegment;
+      launch_coo_contract(c->R, a, c->s);
+      GT_HIP(hipEventRecord(c->ev_comp[b], c->s));
+    }
+    reduce_f64(c->err_part.p, (int)(nseg * kErrPartialsPerSegment), c->scalar.p, c->s);
+  }
   double err = 0.0;
   GT_HIP(hipMemcpyAsync(&err, c->scalar.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
   GT_HIP(hipEventRecord(c->ev[4], c->s));
@@ -463,6 +666,27 @@ double compute_fit(gt_ctx *c) {
   GT_HIP(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
   c->tm.ms_fit = ms;
   if (c->norm == 0.0) return 1.0; // reconstruction.cuh:272-273
+  return 1.0 - std::sqrt(err) / c->norm;
+}
+
+// One ALS iteration (update.cuh:146-198 then reconstruction.cuh:228-277).
+// The fit comes from the last mode's row solves: after that update,
+// sum_e (x_e - xhat_e)^2 = |x|^2 + sum_i (a_i^T G_i a_i - 2 a_i^T C_i), so no
+// reconstruction pass over the nonzeros is needed.
+double iteration(gt_ctx *c) {
+  for (int n = 0; n + 1 < c->N; ++n) mode_update(c, n);
+  const uint64_t parts = mode_update(c, c->N - 1, true);
+  GT_HIP(hipEventRecord(c->ev[3], c->s));
+  reduce_f64(c->fit_part.p, (int)parts, c->scalar.p, c->s);
+  double d = 0.0;
+  GT_HIP(hipMemcpyAsync(&d, c->scalar.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
+  GT_HIP(hipEventRecord(c->ev[4], c->s));
+  GT_HIP(hipStreamSynchronize(c->s));
+  float ms = 0.f;
+  GT_HIP(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+  c->tm.ms_fit = ms;
+  if (c->norm == 0.0) return 1.0; // reconstruction.cuh:272-273
+  const double err = std::max(0.0, c->norm * c->norm + d);
   return 1.0 - std::sqrt(err) / c->norm;
 }
 
@@ -581,6 +805,7 @@ void do_partition(gt_ctx *c) {
   c->have_coo = false;
   c->partitioned = true;
   ensure_work(c);
+  maybe_stream(c);
   c->ms_partition =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->tm.ms_partition = c->ms_partition;
@@ -748,6 +973,7 @@ int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const floa
     require(nnz < (1ull << 32) - 1, GT_ERR_ARG, "nnz must be < 2^32 (uint32 positions)");
     for (int k = 0; k < ctx->N; ++k) require(dims[k] > 0, GT_ERR_ARG, "zero dimension");
     for (int k = 0; k < ctx->N; ++k) require(idx[k] != nullptr || nnz == 0, GT_ERR_ARG, "null idx");
+    ctx->release_host_streams();
     ctx->partitioned = false;
     ctx->initialized = false;
     ctx->nnz = nnz;
@@ -802,6 +1028,8 @@ int gt_get_partition_info(gt_ctx *ctx, gt_partition_info *info) {
     info->partition_type = ctx->plan.type;
     info->stream_count = ctx->plan.streams;
     info->avail_nnz_per_task = ctx->plan.avail;
+    info->out_of_core = ctx->streaming ? 1 : 0;
+    info->segment_nnz = ctx->seg_nnz;
   });
 }
 
@@ -816,6 +1044,7 @@ int gt_get_block_nnz(gt_ctx *ctx, uint64_t *block_nnz_out) {
 int gt_get_block(gt_ctx *ctx, uint64_t block, uint32_t *const *idx, float *vals) {
   return guarded(ctx, [&] {
     require(ctx->partitioned, GT_ERR_STATE, "not partitioned");
+    require(!ctx->streaming, GT_ERR_STATE, "the block COO is not resident (out-of-core mode)");
     require(block < ctx->B, GT_ERR_ARG, "block out of range");
     const uint64_t off = ctx->block_off[block], n = ctx->block_nnz[block];
     if (n == 0) return;
@@ -843,6 +1072,7 @@ int gt_get_block(gt_ctx *ctx, uint64_t block, uint32_t *const *idx, float *vals)
 int gt_partition_dump(gt_ctx *ctx, uint64_t block, int mode, uint64_t *row_ptr, uint32_t *where) {
   return guarded(ctx, [&] {
     require(ctx->partitioned, GT_ERR_STATE, "not partitioned");
+    require(!ctx->streaming, GT_ERR_STATE, "the block COO is not resident (out-of-core mode)");
     require(block < ctx->B && mode >= 0 && mode < ctx->N, GT_ERR_ARG, "block/mode out of range");
     const uint64_t off = ctx->block_off[block], n = ctx->block_nnz[block];
     const uint32_t bd = ctx->plan.bd[mode];
@@ -914,8 +1144,7 @@ int gt_fit(gt_ctx *ctx, double *fit) {
 
 int gt_iteration(gt_ctx *ctx, double *fit) {
   return guarded(ctx, [&] {
-    for (int n = 0; n < ctx->N; ++n) mode_update(ctx, n);
-    double f = compute_fit(ctx);
+    double f = iteration(ctx);
     if (fit) *fit = f;
   });
 }
@@ -925,8 +1154,7 @@ int gt_run(gt_ctx *ctx, int *iters_done, double *fits) {
     int iter = 0;
     double p_fit = -1.0, fit = -1.0;
     for (;;) { // tucker.cuh:166-187
-      for (int n = 0; n < ctx->N; ++n) mode_update(ctx, n);
-      fit = compute_fit(ctx);
+      fit = iteration(ctx);
       if (fits) fits[iter] = fit;
       ++iter;
       if (iter >= ctx->o.max_iters || (p_fit != -1.0 && std::fabs(p_fit - fit) <= ctx->o.tol))
@@ -962,6 +1190,7 @@ int gt_set_factor(gt_ctx *ctx, int mode, const float *in) {
 int gt_get_delta(gt_ctx *ctx, int mode, float *out) {
   return guarded(ctx, [&] {
     require(ctx->partitioned && ctx->initialized, GT_ERR_STATE, "partition and init first");
+    require(!ctx->streaming, GT_ERR_STATE, "the block COO is not resident (out-of-core mode)");
     require(mode >= 0 && mode < ctx->N && out, GT_ERR_ARG, "bad argument");
     DevBuf<float> d;
     d.alloc(ctx->nnz * ctx->R);
@@ -977,6 +1206,7 @@ int gt_get_delta(gt_ctx *ctx, int mode, float *out) {
 int gt_get_recon(gt_ctx *ctx, float *out) {
   return guarded(ctx, [&] {
     require(ctx->partitioned && ctx->initialized, GT_ERR_STATE, "partition and init first");
+    require(!ctx->streaming, GT_ERR_STATE, "the block COO is not resident (out-of-core mode)");
     require(out, GT_ERR_ARG, "bad argument");
     DevBuf<float> x;
     x.alloc(ctx->nnz);

@@ -33,8 +33,8 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
   __shared__ uint32_t s_row[4][kWave];
   extern __shared__ float s_rows_dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const uint64_t c = (uint64_t)blockIdx.x * 4 + w;
-  if (c >= a.nchunks) return;
+  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * 4 + w;
+  if (c >= a.chunk_end) return;
   float *lrows = s_rows_dyn + (size_t)w * contract_lds_rows<R>(a.slots) * R * kWave;
   float *tile = s_tile[w];
   const uint64_t cb = c * a.chunk;
@@ -69,11 +69,15 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
 
   uint32_t cur = __builtin_amdgcn_readfirstlane(rows[cb]);
   uint64_t run_begin = cb;
+  // Runs other than the chunk's first and last are whole rows.
+  const bool head_open = a.row_ptr[cur] < cb;
+  const uint32_t last_row = __builtin_amdgcn_readfirstlane(rows[ce - 1]);
+  const bool tail_open = a.row_ptr[last_row + 1] > ce;
 
   auto flush = [&](uint32_t row, uint64_t b, uint64_t e_end) {
-    const uint64_t rb = a.row_ptr[row], re = a.row_ptr[row + 1];
+    const bool open = (b == cb && head_open) || (e_end == ce && tail_open);
     double *dst;
-    if (b == rb && e_end == re)
+    if (!open)
       dst = a.gram + (size_t)row * W;
     else if (b == cb)
       dst = a.bpart0 + (size_t)c * W;
@@ -118,38 +122,87 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
     wave_sync();
   }
   flush(cur, run_begin, ce);
+  if (lane == 0) a.chunk_tail[c] = tail_open ? last_row : 0xFFFFFFFFu;
 }
 
-// One workgroup per chunk boundary whose row continues into the next chunk
-// and starts in this chunk: sums the partials of that row over every chunk
-// it touches, in chunk order, and writes its Gram row.
-template <int R>
-__global__ void k_gram_fixup(GramArgs a) {
-  constexpr int W = gram_width(R);
-  constexpr int GROUPS = (1024 / W) > 0 ? (1024 / W) : 1;
-  __shared__ double red[GROUPS * W];
-  const uint64_t c = blockIdx.x;
-  const uint64_t nxt = (c + 1) * a.chunk;
-  if (nxt >= a.st.nnz) return;
-  const uint32_t *rows = a.idx_row;
-  const uint32_t r = rows[nxt - 1];
-  if (rows[nxt] != r) return;
-  const uint64_t rb = a.row_ptr[r];
-  if (rb < c * a.chunk) return; // not the row's first chunk
-  const uint64_t last = (a.row_ptr[r + 1] - 1) / a.chunk;
-  const int tid = threadIdx.x;
-  const int q = tid % W, g = tid / W;
-  double s = 0.0;
-  if (g < GROUPS) {
-    if (g == 0) s = (rb == c * a.chunk ? a.bpart0 : a.bpart1)[(size_t)c * W + q];
-    for (uint64_t cc = c + 1 + g; cc <= last; cc += GROUPS) s += a.bpart0[(size_t)cc * W + q];
-    red[g * W + q] = s;
+// Fix-up of rows that straddle chunks, in two levels so that a row spread
+// over thousands of chunks is summed by many waves.  chunk_tail[c] names the
+// row whose run continues from chunk c into chunk c + 1.  The partial of a
+// row's run in chunk c sits in bpart0[c] when the run is the chunk's first
+// (the row starts at or before the chunk's first position), else in
+// bpart1[c].
+//
+// Level 1, one wave per piece of 64 chunks: for each row whose run goes on
+// from chunk c into later chunks of the same piece, the partials of those
+// later chunks (always bpart0) are added into c's slot.
+// Level 2, one wave per piece: for each row that starts in chunk c and
+// continues past it, c's slot (now the row's sum over its first piece) plus
+// bpart0 of the first chunk of each later piece the row covers (the row's
+// sum over that piece) is the row's Gram entry.  Every sum runs in a fixed
+// order, so results are deterministic.
+constexpr int kPiece = 64; // chunks per first-level piece (one per lane)
+
+__device__ inline double sum_bpart0(const double *bp, uint64_t first, uint64_t last, uint64_t step,
+                                    int W, int q) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0; // four independent chains
+  uint64_t k = first;
+  for (; k + 3 * step <= last; k += 4 * step) {
+    s0 += bp[k * W + q];
+    s1 += bp[(k + step) * W + q];
+    s2 += bp[(k + 2 * step) * W + q];
+    s3 += bp[(k + 3 * step) * W + q];
   }
-  __syncthreads();
-  if (g == 0) {
-    double t = red[q];
-    for (int gg = 1; gg < GROUPS; ++gg) t += red[gg * W + q];
-    a.gram[(size_t)r * W + q] = t;
+  for (; k <= last; k += step) s0 += bp[k * W + q];
+  return (s0 + s1) + (s2 + s3);
+}
+
+template <int R>
+__global__ void __launch_bounds__(kWave) k_gram_fix_pieces(GramArgs a) {
+  constexpr int W = gram_width(R);
+  const int lane = threadIdx.x;
+  const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
+  const uint32_t none = 0xFFFFFFFFu;
+  const uint32_t t = c < a.nchunks ? a.chunk_tail[c] : none;               // c -> c + 1
+  const uint32_t pv = (c > 0 && c < a.nchunks) ? a.chunk_tail[c - 1] : none; // c - 1 -> c
+  uint64_t starts = __ballot(t != none && (lane == 0 || pv != t));
+  while (starts) {
+    const int j = __builtin_ctzll(starts);
+    starts &= starts - 1;
+    const uint32_t r = __builtin_amdgcn_readlane(t, j);
+    const uint64_t bal = __ballot(pv == r); // chunks the row runs into from the one before
+    const uint64_t into = j >= kPiece - 1 ? 0ull : bal >> (j + 1);
+    const int e = j + (int)__builtin_ctzll(~into);
+    if (e == j) continue;
+    const uint64_t cj = (uint64_t)blockIdx.x * kPiece + j;
+    double *first = (a.row_ptr[r] <= cj * a.chunk ? a.bpart0 : a.bpart1) + cj * W;
+    for (int q = lane; q < W; q += kWave)
+      first[q] += sum_bpart0(a.bpart0, cj + 1, cj + (e - j), 1, W, q);
+  }
+}
+
+template <int R>
+__global__ void __launch_bounds__(kWave) k_gram_fix_rows(GramArgs a) {
+  constexpr int W = gram_width(R);
+  const int lane = threadIdx.x;
+  const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
+  const uint32_t none = 0xFFFFFFFFu;
+  const uint32_t t = c < a.nchunks ? a.chunk_tail[c] : none;
+  const bool begins = t != none && a.row_ptr[t] >= c * a.chunk; // the row starts in chunk c
+  uint64_t mask = __ballot(begins);
+  while (mask) {
+    const int j = __builtin_ctzll(mask);
+    mask &= mask - 1;
+    const uint32_t r = __builtin_amdgcn_readlane(t, j);
+    const uint64_t cj = (uint64_t)blockIdx.x * kPiece + j;
+    const uint64_t rb = a.row_ptr[r];
+    const uint64_t last = (a.row_ptr[r + 1] - 1) / a.chunk;
+    const double *first = (rb == cj * a.chunk ? a.bpart0 : a.bpart1) + cj * W;
+    const uint64_t p0 = cj / kPiece, pl = last / kPiece;
+    for (int q = lane; q < W; q += kWave) {
+      double s = first[q];
+      if (pl > p0) s += sum_bpart0(a.bpart0, (p0 + 1) * kPiece, pl * kPiece, kPiece, W, q);
+      a.gram[(size_t)r * W + q] = s;
+    }
   }
 }
 
@@ -200,9 +253,14 @@ __global__ void __launch_bounds__(256) k_coo_contract(CooArgs a) {
 
 __global__ void k_reduce_f64(const double *__restrict__ in, int n, double *__restrict__ out) {
   __shared__ double red[256];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) s += in[i];
-  red[threadIdx.x] = s;
+  double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0}; // independent chains
+  int i = threadIdx.x;
+  for (; i + 7 * 256 < n; i += 8 * 256) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] += in[i + k * 256];
+  }
+  for (; i < n; i += 256) s[0] += in[i];
+  red[threadIdx.x] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
@@ -212,18 +270,17 @@ __global__ void k_reduce_f64(const double *__restrict__ in, int n, double *__res
 }
 
 template <int R> void gram_fp32_impl(const GramArgs &a, hipStream_t s) {
-  if (a.nchunks == 0) return;
-  const unsigned grid = (unsigned)((a.nchunks + 3) / 4);
+  if (a.chunk_end <= a.chunk_begin) return;
+  const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + 3) / 4);
   const size_t lds = (size_t)4 * contract_lds_rows<R>(a.slots) * R * kWave * sizeof(float);
   hipLaunchKernelGGL(k_gram_fp32<R>, dim3(grid), dim3(256), lds, s, a);
 }
 
 template <int R> void gram_fixup_impl(const GramArgs &a, hipStream_t s) {
   if (a.nchunks <= 1) return;
-  constexpr int W = gram_width(R);
-  constexpr int GROUPS = (1024 / W) > 0 ? (1024 / W) : 1;
-  hipLaunchKernelGGL(k_gram_fixup<R>, dim3((unsigned)(a.nchunks - 1)), dim3(GROUPS * W), 0, s,
-                     a);
+  const unsigned grid = (unsigned)((a.nchunks + kPiece - 1) / kPiece);
+  hipLaunchKernelGGL(k_gram_fix_pieces<R>, dim3(grid), dim3(kWave), 0, s, a);
+  hipLaunchKernelGGL(k_gram_fix_rows<R>, dim3(grid), dim3(kWave), 0, s, a);
 }
 
 template <int R> int coo_impl(const CooArgs &a, hipStream_t s) {

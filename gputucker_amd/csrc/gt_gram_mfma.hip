@@ -45,6 +45,8 @@ namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 enum DeltaKind { kValu = 0, kBf16 = 1, kBf16R = 2 };
@@ -79,19 +81,34 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
   __shared__ uint32_t s_row[4][kWave];
   __shared__ uint32_t s_iL[4][kWave];
   __shared__ float s_x[4][kWave];
-  extern __shared__ float dyn[];
+  extern __shared__ __align__(16) float dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const int slots = a.slots;
-  // the core operand table (BF16 kinds) goes to LDS once per workgroup
-  s4 *s_gb = reinterpret_cast<s4 *>(dyn + (size_t)wpb * lds_floats_per_wave<KIND>(slots, R, O));
+  // the core operand tables (BF16 kinds) go to LDS once per workgroup:
+  // kBf16: K16 hi table; kBf16R: K32 table, then (R > 10) the K16 lo table
+  uint4 *s_tab = reinterpret_cast<uint4 *>(dyn + (size_t)wpb * lds_floats_per_wave<KIND>(slots, R, O));
+  const s4 *t16 = reinterpret_cast<const s4 *>(s_tab);
+  const s8 *t32 = reinterpret_cast<const s8 *>(s_tab);
+  const s4 *t16lo = reinterpret_cast<const s4 *>(s_tab + (size_t)O * kWave);
   if constexpr (KIND != kValu) {
-    const int tab = (KIND == kBf16R ? 2 : 1) * O * kWave;
-    const s4 *src = reinterpret_cast<const s4 *>(a.Gb);
-    for (int i = threadIdx.x; i < tab; i += blockDim.x) s_gb[i] = src[i];
+    const uint4 *src = reinterpret_cast<const uint4 *>(a.Gb);
+    if constexpr (KIND == kBf16) {
+      const int n = (int)(gb_k16_words(O) / 8);
+      for (int i = threadIdx.x; i < n; i += blockDim.x) s_tab[i] = src[i];
+    } else {
+      const uint4 *k32 = src + gb_k16_words(O) / 8;
+      const int n = (int)(gb_k32_words(O) / 8);
+      for (int i = threadIdx.x; i < n; i += blockDim.x) s_tab[i] = k32[i];
+      if constexpr (!bf16r_single_mfma(R)) {
+        const uint4 *lo = k32 + n;
+        const int n2 = (int)(gb_k16_words(O) / 8);
+        for (int i = threadIdx.x; i < n2; i += blockDim.x) s_tab[n + i] = lo[i];
+      }
+    }
     __syncthreads();
   }
-  const uint64_t c = (uint64_t)blockIdx.x * wpb + w;
-  if (c >= a.nchunks) return;
+  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
+  if (c >= a.chunk_end) return;
   float *lrows = dyn + (size_t)w * lds_floats_per_wave<KIND>(slots, R, O);
   float *souter = lrows + (slots >= 2 ? (size_t)slots * R * kWave : 0);
   float *tile = s_tile[w];
@@ -107,11 +124,17 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
   d4 acc64 = {0.0, 0.0, 0.0, 0.0};
   uint32_t cur = __builtin_amdgcn_readfirstlane(a.idx_row[cb]);
   uint64_t run_begin = cb;
+  // Does the chunk's first row start before it, or its last row end after it?
+  // Two row-pointer reads per chunk; inside the chunk every other run is
+  // complete, so flushes need no memory reads.
+  const bool head_open = a.row_ptr[cur] < cb;
+  const uint32_t last_row = __builtin_amdgcn_readfirstlane(a.idx_row[ce - 1]);
+  const bool tail_open = a.row_ptr[last_row + 1] > ce;
 
   auto flush = [&](uint32_t row, uint64_t b, uint64_t e_end) {
-    const uint64_t rb = a.row_ptr[row], re = a.row_ptr[row + 1];
+    const bool open = (b == cb && head_open) || (e_end == ce && tail_open);
     double *dst;
-    if (b == rb && e_end == re)
+    if (!open)
       dst = a.gram + (size_t)row * W;
     else if (b == cb)
       dst = a.bpart0 + (size_t)c * W;
@@ -122,22 +145,34 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
       if (pk[r] >= 0) dst[pk[r]] = acc64[r];
   };
 
-  for (uint64_t tb = cb; tb < ce; tb += kWave) {
+  // stream indices of the next tile are prefetched behind the current
+  // tile's gathers, so each tile waits for one memory round trip, not three
+  uint32_t n_row, n_iL, n_io[kSlots];
+  float n_x;
+  auto load_idx = [&](uint64_t tb) {
     const uint64_t e = tb + lane;
-    const bool valid = e < ce;
-    const uint64_t ee = valid ? e : cb; // in-bounds position for padding lanes
+    const uint64_t ee = e < ce ? e : cb; // in-bounds position for padding lanes
+    n_row = a.idx_row[ee];
+    n_iL = a.idx_L[ee];
+#pragma unroll
+    for (int t = 0; t < kSlots; ++t) n_io[t] = (t < slots) ? a.idx_O[t][ee] : 0u;
+    n_x = e < ce ? a.st.val[ee] : 0.f;
+  };
+  load_idx(cb);
+
+  for (uint64_t tb = cb; tb < ce; tb += kWave) {
+    const bool valid = tb + lane < ce;
     const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
-    // ---- coalesced stream reads, then every gather of the tile
-    const uint32_t row = a.idx_row[ee];
-    const uint32_t iL = a.idx_L[ee];
+    const uint32_t row = n_row, iL = n_iL;
     uint32_t io[kSlots];
 #pragma unroll
-    for (int t = 0; t < kSlots; ++t) io[t] = (t < slots) ? a.idx_O[t][ee] : 0u;
-    const float x = valid ? a.st.val[ee] : 0.f;
+    for (int t = 0; t < kSlots; ++t) io[t] = n_io[t];
+    const float x = n_x;
     s_row[w][lane] = valid ? row : 0xFFFFFFFFu;
     float v[4][4];
     if constexpr (KIND == kValu) {
       float d[R];
+      if (tb + kWave < ce) load_idx(tb + kWave);
       contract_slots<R>(slots, a.A_L, iL, a.A_O, io, a.Gt, lrows, lane, d);
 #pragma unroll
       for (int r = 0; r < 16; ++r)
@@ -148,7 +183,24 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[mb][i] = tile[(mb * 16 + 4 * g + i) * kTP + q];
     } else {
-      // outer rows of this lane's nonzero
+      s_iL[w][lane] = iL;
+      s_x[w][lane] = x;
+      wave_sync();
+      // every gather of the tile: the A_L row words of the 16 nonzeros each
+      // MFMA block needs (per the lane group's plan), then this lane's outer rows
+      const int plan = (KIND == kBf16R) ? bf16r_plan(R, g) : (int)kPlanHL;
+      const int off0 = plan == kPlanHH ? 0 : (plan == kPlanR10 ? 8 : 4 * g);
+      const bool has0 = plan != kPlanZero && off0 < RS;
+      const bool has1 = plan == kPlanHH && RS >= 8;
+      f4 f0[4], f1[4];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const float *rowp = a.A_L + (size_t)s_iL[w][mb * 16 + q] * RS;
+        f0[mb] = f4{0.f, 0.f, 0.f, 0.f};
+        f1[mb] = f4{0.f, 0.f, 0.f, 0.f};
+        if (has0) f0[mb] = *reinterpret_cast<const f4 *>(rowp + off0);
+        if (has1) f1[mb] = *reinterpret_cast<const f4 *>(rowp + 4);
+      }
       float aO[RS];
       if (slots == 1) load_row<R>(a.A_O[0], io[0], aO);
       if (slots >= 2) {
@@ -161,9 +213,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
           for (int j = 0; j < R; ++j) lrows[(t * R + j) * kWave + lane] = rt[j];
         }
       }
+      if (tb + kWave < ce) load_idx(tb + kWave);
       // s_m[o] for this lane's nonzero, into LDS for the stage-2 layout
-      s_iL[w][lane] = iL;
-      s_x[w][lane] = x;
       if (slots == 0) {
         souter[lane] = valid ? 1.f : 0.f;
       } else if (slots == 1) {
@@ -188,43 +239,53 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
           }
         }
       }
-      wave_sync();
-      // A operands: row of A_L for nonzero 16 mb + q, columns 4g..4g+3
-      s4 ahi[4], alo[4];
+      // A operands: kBf16 hi words (K16); kBf16R the plan's K32 words, plus
+      // hi words for the a_hi.G_lo MFMA when R > 10
+      s4 ahi[4];
+      s8 a32[4];
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
-        f4 val = {0.f, 0.f, 0.f, 0.f};
-        if (4 * g < RS)
-          val = *reinterpret_cast<const f4 *>(a.A_L + (size_t)s_iL[w][mb * 16 + q] * RS + 4 * g);
+        s4 lo0, hi1;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          const short h = bf16_bits(val[t]);
+          const short h = bf16_bits(f0[mb][t]);
           ahi[mb][t] = h;
-          alo[mb][t] = bf16_bits(val[t] - bf16_value(h));
+          lo0[t] = bf16_bits(f0[mb][t] - bf16_value(h));
+          hi1[t] = bf16_bits(f1[mb][t]);
+        }
+        if constexpr (KIND == kBf16R) {
+          s4 up = lo0;
+          if (plan == kPlanHH) up = hi1;
+          if (plan == kPlanR10) up = s4{lo0[0], lo0[1], ahi[mb][0], ahi[mb][1]};
+          a32[mb] = s8{ahi[mb][0], ahi[mb][1], ahi[mb][2], ahi[mb][3], up[0], up[1], up[2], up[3]};
         }
       }
+      wave_sync();
       float dacc[4][4];
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
         for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
       auto stage = [&](int o) {
-        const s4 bh = s_gb[o * kWave + lane];
-        s4 bl;
-        if constexpr (KIND == kBf16R) bl = s_gb[(O + o) * kWave + lane];
         f4 u[4];
-#pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-          const f4 z = {0.f, 0.f, 0.f, 0.f};
-          u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[mb], bh, z, 0, 0, 0);
-        }
-        if constexpr (KIND == kBf16R) {
+        const f4 z = {0.f, 0.f, 0.f, 0.f};
+        if constexpr (KIND == kBf16) {
+          const s4 bh = t16[o * kWave + lane];
 #pragma unroll
           for (int mb = 0; mb < 4; ++mb)
-            u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo[mb], bh, u[mb], 0, 0, 0);
+            u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[mb], bh, z, 0, 0, 0);
+        } else {
+          const b8 b = __builtin_bit_cast(b8, t32[o * kWave + lane]);
 #pragma unroll
           for (int mb = 0; mb < 4; ++mb)
-            u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[mb], bl, u[mb], 0, 0, 0);
+            u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a32[mb]), b, z,
+                                                            0, 0, 0);
+          if constexpr (!bf16r_single_mfma(R)) {
+            const s4 bl = t16lo[o * kWave + lane];
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb)
+              u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[mb], bl, u[mb], 0, 0, 0);
+          }
         }
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
@@ -256,7 +317,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
     while (s < cnt) {
       if ((starts >> s) & 1ull) {
         flush(cur, run_begin, tb + s);
-        cur = __builtin_amdgcn_readfirstlane(s_row[w][s]);
+        cur = __builtin_amdgcn_readlane(myrow, s);
         run_begin = tb + s;
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc64[r] = 0.0;
@@ -281,15 +342,24 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
     wave_sync();
   }
   flush(cur, run_begin, ce);
+  if (lane == 0) a.chunk_tail[c] = tail_open ? last_row : 0xFFFFFFFFu;
 }
 
 template <int R, int KIND> void launch_impl(const GramArgs &a, int O, hipStream_t st) {
-  if (a.nchunks == 0) return;
+  if (a.chunk_end <= a.chunk_begin) return;
   const size_t per_wave = lds_floats_per_wave<KIND>(a.slots, R, O) * sizeof(float);
-  const size_t table = KIND == kValu ? 0 : (size_t)(KIND == kBf16R ? 2 : 1) * O * kWave * 8;
+  const size_t table = KIND == kValu    ? 0
+                       : KIND == kBf16  ? gb_k16_words(O) * 2
+                                        : (gb_k32_words(O) + (bf16r_single_mfma(R) ? 0 : gb_k16_words(O))) * 2;
+  if constexpr (KIND != kValu) {
+    if (per_wave + table > 64 * 1024) { // core tables too large for LDS: FP32 VALU delta
+      launch_impl<R, kValu>(a, O, st);
+      return;
+    }
+  }
   int wpb = 4;
   while (wpb > 1 && per_wave * wpb + table > 64 * 1024) --wpb;
-  const unsigned grid = (unsigned)((a.nchunks + wpb - 1) / wpb);
+  const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
   hipLaunchKernelGGL((k_gram_mfma<R, KIND>), dim3(grid), dim3(64 * wpb), per_wave * wpb + table,
                      st, a, wpb, O);
 }

@@ -123,8 +123,12 @@ struct GramArgs {
   double *gram;               // rows x W packed Gram
   double *bpart0;             // chunks x W: first incomplete run of each chunk
   double *bpart1;             // chunks x W: last incomplete run of each chunk
+  uint32_t *chunk_tail;       // chunks: row whose run continues into the next chunk, else ~0u
   uint64_t chunk;             // nonzeros per chunk (multiple of 64)
-  uint64_t nchunks;
+  uint64_t nchunks;           // chunks of the whole stream
+  uint64_t chunk_begin;       // this launch covers chunks [chunk_begin, chunk_end); the
+  uint64_t chunk_end;         // stream pointers must be valid over those positions
+  uint64_t rows;              // rows of this mode (row_ptr has rows + 1 entries)
   const void *Gb;             // BF16 core operand table (hi, then lo): [o][lane][4]
   // the same operands in slot order, so the kernel never indexes by mode:
   // L = last mode other than n, outer t = the other modes ascending
@@ -140,6 +144,33 @@ void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);
 // delta + Gram with the Gram on MFMA (R <= 15); precision selects the delta
 // stage (GT_PREC_FP32: VALU, GT_PREC_BF16/BF16R: MFMA).  O = R^(N-2).
 bool gram_mfma_supported(int R);
+
+// Operand plan of the refined BF16 delta (split BF16: a = a_hi + a_lo,
+// G = G_hi + G_lo, a.G ~ a_hi.G_hi + a_lo.G_hi + a_hi.G_lo) on one
+// v_mfma_f32_16x16x32_bf16 per 16 nonzeros and outer index.  Lane group g
+// (lanes 16g..16g+15) supplies 8 of the 32 K slots; its factor-row words and
+// core words per slot are:
+//   kPlanHL   [hi(a[4g..4g+3]), lo(a[4g..4g+3])]   x [Ghi[4g..], Ghi[4g..]]
+//   kPlanHH   [hi(a[0..3]), hi(a[4..7])]           x [Glo[0..3], Glo[4..7]]
+//   kPlanR10  [hi(a[8..11]), lo(a[8]), lo(a[9]), hi(a[8]), hi(a[9])]
+//                                   x [Ghi[8..11], Ghi[8], Ghi[9], Glo[8], Glo[9]]
+//   kPlanZero zeros
+// R <= 10 fits all three terms in the 32 slots; above that the K=32 MFMA
+// carries the first two terms (kPlanHL everywhere) and a 16x16x16 MFMA adds
+// a_hi.G_lo.
+enum Bf16rPlan { kPlanZero = 0, kPlanHL = 1, kPlanHH = 2, kPlanR10 = 3 };
+__host__ __device__ constexpr bool bf16r_single_mfma(int R) { return R <= 10; }
+__host__ __device__ constexpr int bf16r_plan(int R, int g) {
+  return R >= 11 ? kPlanHL
+       : R >= 9  ? (g < 2 ? kPlanHL : (g == 2 ? kPlanR10 : kPlanHH))
+       : R >= 5  ? (g < 2 ? kPlanHL : (g == 2 ? kPlanHH : kPlanZero))
+                 : (g == 0 ? kPlanHL : (g == 1 ? kPlanHH : kPlanZero));
+}
+// BF16 core operand tables, in uint16 words: K16 hi table ([o][lane][4]),
+// then the K32 refined table ([o][lane][8]), then the K16 lo table
+// ([o][lane][4]; used when R > 10).
+__host__ __device__ constexpr size_t gb_k16_words(int O) { return (size_t)O * kWave * 4; }
+__host__ __device__ constexpr size_t gb_k32_words(int O) { return (size_t)O * kWave * 8; }
 void launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
 
 struct SolveArgs {
@@ -148,8 +179,11 @@ struct SolveArgs {
   uint64_t rows;
   double lambda;
   float *A; // rows x R, written
+  double *fit_partials; // null, or solve_partials(R, rows) residual partials (FP64)
 };
-void launch_solve(int R, const SolveArgs &a, hipStream_t s);
+// Number of fit partials launch_solve writes for `rows` rows.
+uint64_t solve_partials(int R, uint64_t rows);
+uint64_t launch_solve(int R, const SolveArgs &a, hipStream_t s); // returns partials written
 
 // Contraction over block-ordered COO for one mode n, operands in slot order
 // (as GramArgs): delta of every nonzero (delta_out), xhat = A_n[i_n] . delta

@@ -7,6 +7,14 @@
 // one lane per row for R <= 16 (the factor lives in registers), one wave per
 // row above that (the factor lives in LDS).  Rows without nonzeros have
 // B = lambda I and C = 0, so their solution is 0 (the reference's result too).
+//
+// With fit_partials set, each solve also yields the row's share of the
+// residual, sum_e (x_e - a_i . delta_e)^2 - sum_e x_e^2 = a^T G a - 2 a^T C
+// with G = sum delta delta^T = L L^T - lambda I, evaluated in FP64 for a
+// rounded to FP32 as stored.  After the last mode of an ALS iteration delta is
+// the contraction with every other (already updated) factor, so a_i . delta_e
+// is the reconstruction of reconstruction.cuh:58-81 and the fit needs no
+// pass over the nonzeros.
 #include "gt_internal.hpp"
 
 namespace gt {
@@ -21,54 +29,78 @@ template <int R>
 __global__ void __launch_bounds__(64) k_solve_lane(SolveArgs a) {
   constexpr int W = gram_width(R);
   const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= a.rows) return;
-  float *out = a.A + row * row_stride(R);
-  if (a.row_ptr[row + 1] == a.row_ptr[row]) {
+  double contrib = 0.0;
+  if (row < a.rows) {
+    float *out = a.A + row * row_stride(R);
+    if (a.row_ptr[row + 1] == a.row_ptr[row]) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) out[r] = 0.f;
-    return;
-  }
-  const double *g = a.gram + row * W;
-  double L[R * (R + 1) / 2]; // packed lower: L(i, j) at i(i+1)/2 + j
+      for (int r = 0; r < R; ++r) out[r] = 0.f;
+    } else {
+      const double *g = a.gram + row * W;
+      double L[R * (R + 1) / 2]; // packed lower: L(i, j) at i(i+1)/2 + j
 #pragma unroll
-  for (int i = 0; i < R; ++i)
+      for (int i = 0; i < R; ++i)
 #pragma unroll
-    for (int j = 0; j <= i; ++j) L[i * (i + 1) / 2 + j] = g[up_index(R, j, i)];
+        for (int j = 0; j <= i; ++j) L[i * (i + 1) / 2 + j] = g[up_index(R, j, i)];
+      double cv[R];
 #pragma unroll
-  for (int i = 0; i < R; ++i) L[i * (i + 1) / 2 + i] += a.lambda;
+      for (int i = 0; i < R; ++i) cv[i] = g[R * (R + 1) / 2 + i];
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    double dj = L[j * (j + 1) / 2 + j];
+      for (int i = 0; i < R; ++i) L[i * (i + 1) / 2 + i] += a.lambda;
 #pragma unroll
-    for (int k = 0; k < j; ++k) dj -= L[j * (j + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
-    dj = sqrt(dj);
-    L[j * (j + 1) / 2 + j] = dj;
-    const double inv = 1.0 / dj;
+      for (int j = 0; j < R; ++j) {
+        double dj = L[j * (j + 1) / 2 + j];
 #pragma unroll
-    for (int i = j + 1; i < R; ++i) {
-      double v = L[i * (i + 1) / 2 + j];
+        for (int k = 0; k < j; ++k) dj -= L[j * (j + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
+        dj = sqrt(dj);
+        L[j * (j + 1) / 2 + j] = dj;
+        const double inv = 1.0 / dj;
 #pragma unroll
-      for (int k = 0; k < j; ++k) v -= L[i * (i + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
-      L[i * (i + 1) / 2 + j] = v * inv;
+        for (int i = j + 1; i < R; ++i) {
+          double v = L[i * (i + 1) / 2 + j];
+#pragma unroll
+          for (int k = 0; k < j; ++k) v -= L[i * (i + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
+          L[i * (i + 1) / 2 + j] = v * inv;
+        }
+      }
+      double y[R];
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        double v = cv[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) v -= L[i * (i + 1) / 2 + k] * y[k];
+        y[i] = v / L[i * (i + 1) / 2 + i];
+      }
+#pragma unroll
+      for (int i = R - 1; i >= 0; --i) {
+        double v = y[i];
+#pragma unroll
+        for (int k = i + 1; k < R; ++k) v -= L[k * (k + 1) / 2 + i] * y[k];
+        y[i] = v / L[i * (i + 1) / 2 + i];
+      }
+      float af[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        af[r] = (float)y[r];
+        out[r] = af[r];
+      }
+      if (a.fit_partials) { // a^T (L L^T) a - lambda |a|^2 - 2 a^T c
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+          double v = 0.0;
+#pragma unroll
+          for (int k = i; k < R; ++k) v += L[k * (k + 1) / 2 + i] * (double)af[k];
+          const double ai = (double)af[i];
+          contrib += v * v - a.lambda * ai * ai - 2.0 * ai * cv[i];
+        }
+      }
     }
   }
-  double y[R];
+  if (a.fit_partials) {
 #pragma unroll
-  for (int i = 0; i < R; ++i) {
-    double v = g[R * (R + 1) / 2 + i];
-#pragma unroll
-    for (int k = 0; k < i; ++k) v -= L[i * (i + 1) / 2 + k] * y[k];
-    y[i] = v / L[i * (i + 1) / 2 + i];
+    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
+    if (threadIdx.x == 0) a.fit_partials[blockIdx.x] = contrib;
   }
-#pragma unroll
-  for (int i = R - 1; i >= 0; --i) {
-    double v = y[i];
-#pragma unroll
-    for (int k = i + 1; k < R; ++k) v -= L[k * (k + 1) / 2 + i] * y[k];
-    y[i] = v / L[i * (i + 1) / 2 + i];
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r) out[r] = (float)y[r];
 }
 
 // One wave per row; lane i owns row i of the factor (R <= 64).
@@ -82,13 +114,16 @@ __global__ void __launch_bounds__(64) k_solve_wave(SolveArgs a) {
   float *out = a.A + row * row_stride(R);
   if (a.row_ptr[row + 1] == a.row_ptr[row]) {
     if (lane < R) out[lane] = 0.f;
+    if (a.fit_partials && lane == 0) a.fit_partials[row] = 0.0;
     return;
   }
   const double *g = a.gram + row * W;
+  double ci = 0.0;
   if (lane < R) {
     for (int j = 0; j <= lane; ++j) M[lane][j] = g[up_index(R, j, lane)];
     M[lane][lane] += a.lambda;
-    y[lane] = g[R * (R + 1) / 2 + lane];
+    ci = g[R * (R + 1) / 2 + lane];
+    y[lane] = ci;
   }
   __syncthreads();
   for (int j = 0; j < R; ++j) { // right-looking Cholesky
@@ -113,24 +148,44 @@ __global__ void __launch_bounds__(64) k_solve_wave(SolveArgs a) {
     __syncthreads();
   }
   if (lane < R) out[lane] = (float)y[lane];
+  if (a.fit_partials) { // a^T (L L^T) a - lambda |a|^2 - 2 a^T c, lane i: (L^T a)_i
+    double contrib = 0.0;
+    if (lane < R) {
+      double v = 0.0;
+      for (int k = lane; k < R; ++k) v += M[k][lane] * (double)(float)y[k];
+      const double ai = (double)(float)y[lane];
+      contrib = v * v - a.lambda * ai * ai - 2.0 * ai * ci;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
+    if (lane == 0) a.fit_partials[row] = contrib;
+  }
 }
 
-template <int R> void solve_impl(const SolveArgs &a, hipStream_t s) {
-  if (a.rows == 0) return;
+template <int R> uint64_t solve_impl(const SolveArgs &a, hipStream_t s) {
+  if (a.rows == 0) return 0;
   if constexpr (R <= 16) {
-    hipLaunchKernelGGL(k_solve_lane<R>, dim3((unsigned)((a.rows + 63) / 64)), dim3(64), 0, s, a);
+    const uint64_t grid = (a.rows + 63) / 64;
+    hipLaunchKernelGGL(k_solve_lane<R>, dim3((unsigned)grid), dim3(64), 0, s, a);
+    return grid;
   } else {
     hipLaunchKernelGGL(k_solve_wave<R>, dim3((unsigned)a.rows), dim3(64), 0, s, a);
+    return a.rows;
   }
 }
 
 } // namespace
 
-void launch_solve(int R, const SolveArgs &a, hipStream_t s) {
+uint64_t solve_partials(int R, uint64_t rows) {
+  return R <= 16 ? (rows + 63) / 64 : rows;
+}
+
+uint64_t launch_solve(int R, const SolveArgs &a, hipStream_t s) {
+  uint64_t parts = 0;
   switch (R) {
 #define GT_CASE(r)                                                                               \
   case r:                                                                                        \
-    solve_impl<r>(a, s);                                                                         \
+    parts = solve_impl<r>(a, s);                                                                 \
     break;
     GT_FOR_EACH_RANK(GT_CASE)
 #undef GT_CASE
@@ -138,6 +193,7 @@ void launch_solve(int R, const SolveArgs &a, hipStream_t s) {
     throw std::invalid_argument("rank not built");
   }
   GT_HIP(hipGetLastError());
+  return parts;
 }
 
 } // namespace gt

@@ -57,7 +57,9 @@ typedef struct gt_opts {
   double tol;                       /* fit tolerance (tucker.cuh:183, default (double)0.0001f) */
   int precision;                    /* gt_precision */
   uint32_t partition[GT_MAX_ORDER]; /* forced P_n; all zero = planner (optimizer.tpp:70-164) */
-  uint64_t device_budget;           /* bytes the planner may assume; 0 = free device memory */
+  uint64_t device_budget;           /* bytes the planner may assume; 0 = free device memory.
+                                       When the resident layout exceeds it, the per-mode
+                                       streams are staged from pinned host memory */
   int chunk_tiles;                  /* 64-nonzero tiles per Gram chunk; 0 = autotune */
   int profile;                      /* 1: time the Gram kernel with HIP events */
 } gt_opts;
@@ -75,6 +77,8 @@ typedef struct gt_partition_info {
   int partition_type;        /* 0 dimension-based, 1 nonzero-based (constants.hpp:38-42) */
   int stream_count;          /* optimizer.tpp:130,140 */
   uint64_t avail_nnz_per_task;
+  int out_of_core;           /* 1: streams staged from pinned host memory per mode update */
+  uint64_t segment_nnz;      /* nonzeros per staged segment (out-of-core) */
 } gt_partition_info;
 
 typedef struct gt_timings {
