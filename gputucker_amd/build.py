@@ -28,9 +28,13 @@ ARCH = os.environ.get("GT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops off: packed FP32 VALU forms are not used on this hardware.
+# -amdgpu-mfma-vgpr-form: MFMA results land in VGPRs; the delta kernels feed
+# every accumulator element to VALU code, which would otherwise cost one
+# v_accvgpr_read per element.
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument", "-Wno-unknown-warning-option",
     "-I", os.path.join(ROOT, "include"),
 ]

@@ -26,18 +26,18 @@
 // fly; the B operand (the core, zero-padded to 16 x 16 per o) comes from a
 // small table laid out in lane order.
 //
-// Gram on MFMA (16x16x4 f64).  After the delta stage lane l holds
-// v[mb][i] = delta[m][q] with m = 16 mb + 4 (l/16) + i and q = l % 16, with
-// x[m] in column q = R and zeros beyond.  For a fixed (mb, i) the four lane
-// groups hold four different nonzeros, so one MFMA with v as both operands
-// adds sum_m v[m][p] v[m][q] for those four nonzeros: the upper triangle is
-// B = sum delta delta^T, column R is C = sum x delta.  Each run of equal rows
-// inside the tile gets its own masked MFMAs.  The FP32 deltas enter as FP64,
-// so every product is exact and every sum is FP64: B is positive definite to
-// FP64 rounding even for rows with fewer than R nonzeros, where FP32-rounded
-// products would shift the zero eigenvalues of sum dd^T by about lambda.
-// The accumulators are flushed on row change (complete rows to the Gram
-// table, chunk-straddling runs to the partial tables for the fix-up).
+// Gram on MFMA (v_mfma_f64_4x4x4f64).  The deltas of a 64-nonzero tile and
+// the values (v = [delta, x], zero-padded to whole 4-column tiles) go to LDS
+// column-major; per group of 16 nonzeros each 4x4 block pair (ta <= tc) of
+// the upper triangle of sum v v^T takes one MFMA, the four 4x4x4 blocks of
+// the instruction holding four nonzeros each.  Column R of the result is
+// C = sum x delta.  Each run of equal rows inside the tile gets its own
+// masked MFMAs.  The FP32 values enter as FP64, so every product is exact
+// and every sum is FP64: B is positive definite to FP64 rounding even for
+// rows with fewer than R nonzeros, where FP32-rounded products would shift
+// the zero eigenvalues of sum dd^T by about lambda.  The accumulators are
+// flushed on row change (complete rows to the Gram table, chunk-straddling
+// runs to the partial tables for the fix-up).
 #include "gt_contract.hpp"
 
 namespace gt {
@@ -47,19 +47,32 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef short s8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
-typedef double d4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float fv2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 enum DeltaKind { kValu = 0, kBf16 = 1, kBf16R = 2 };
-constexpr int kTP = 17; // LDS pitch of a delta tile row (odd: fewer bank conflicts)
+// The delta tile of 64 nonzeros sits in LDS column-major, tileT[col][m]
+// (col < 16: delta columns < R, x in column R, zeros above), with a column
+// pitch of 72 floats so the Gram operand reads hit distinct banks.
+constexpr int kDP = 72;
 
-__device__ inline short bf16_bits(float x) { // round to nearest even
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+// DPP row rotation (within each row of 16 lanes) of an FP64 value
+template <int CTRL> __device__ inline double dpp_f64(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)u, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = __builtin_amdgcn_update_dpp(0u, (uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
-__device__ inline float bf16_value(short h) {
-  return __uint_as_float(((uint32_t)(uint16_t)h) << 16);
+constexpr int kDppRowRor4 = 0x124, kDppRowRor8 = 0x128;
+
+// two floats to a BF16 word pair (a in the low half), round to nearest even
+__device__ inline uint32_t pk_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((fv2{a, b}), bf2));
 }
+__device__ inline float bf16_low_f(uint32_t p) { return __uint_as_float(p << 16); }
+__device__ inline float bf16_high_f(uint32_t p) { return __uint_as_float(p & 0xFFFF0000u); }
 
 template <int R> __device__ inline int packed_index(int p, int q) {
   if (p <= q && q < R) return p * R - p * (p - 1) / 2 + (q - p);
@@ -77,10 +90,9 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
   static_assert(R <= 15, "MFMA Gram covers R <= 15");
   constexpr int W = gram_width(R);
   constexpr int RS = row_stride(R);
-  __shared__ float s_tile[4][kWave * kTP];
+  __shared__ float s_tile[4][16 * kDP];
   __shared__ uint32_t s_row[4][kWave];
   __shared__ uint32_t s_iL[4][kWave];
-  __shared__ float s_x[4][kWave];
   extern __shared__ __align__(16) float dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const int slots = a.slots;
@@ -116,12 +128,21 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
   const int q = lane & 15, g = lane >> 4;
 
-  // v_mfma_f64_16x16x4f64 leaves D[p][q] in lane 16 (p % 4) + q, register
-  // p / 4 (rows interleaved across the lane groups)
-  int pk[4];
+  // Gram on v_mfma_f64_4x4x4f64: four 4x4x4 blocks; block beta takes A[i][k]
+  // and B[k][j] from lane 16k + 4beta + i (+ j) and leaves D[i][j] in lane
+  // 16i + 4beta + j.  Lane 16k + 4beta + xx supplies, for column tile t, the
+  // value X_t = tile[4t + xx][m] of nonzero m = 16mb + 4k + beta, so one MFMA
+  // with A = X_ta, B = X_tc adds the (ta, tc) block of sum over those 16
+  // nonzeros of v v^T (v = [delta, x]), one partial per beta; the four
+  // partials are added at flush.  Only the T(T+1)/2 block pairs ta <= tc of
+  // the upper triangle are formed.  Products of FP32 values in FP64 are
+  // exact, every sum is FP64.
+  constexpr int T = (R + 4) / 4; // column tiles covering columns 0..R
+  constexpr int NP = T * (T + 1) / 2;
+  const int gk = lane >> 4, gbeta = (lane >> 2) & 3, gx = lane & 3;
+  double acc[NP];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) pk[r] = packed_index<R>(g + 4 * r, q);
-  d4 acc64 = {0.0, 0.0, 0.0, 0.0};
+  for (int p = 0; p < NP; ++p) acc[p] = 0.0;
   uint32_t cur = __builtin_amdgcn_readfirstlane(a.idx_row[cb]);
   uint64_t run_begin = cb;
   // Does the chunk's first row start before it, or its last row end after it?
@@ -140,9 +161,21 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
       dst = a.bpart0 + (size_t)c * W;
     else
       dst = a.bpart1 + (size_t)c * W;
+    // add the four block partials (lanes 4 apart inside each row of 16);
+    // quad beta then stores the block pairs p with p % 4 == beta
+    const int fi = lane >> 4, fj = lane & 3;
+    int p = 0;
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (pk[r] >= 0) dst[pk[r]] = acc64[r];
+    for (int ta = 0; ta < T; ++ta)
+#pragma unroll
+      for (int tc = ta; tc < T; ++tc, ++p) {
+        double v = acc[p];
+        v += dpp_f64<kDppRowRor8>(v);
+        v += dpp_f64<kDppRowRor4>(v);
+        const int P = 4 * ta + fi, Q = 4 * tc + fj;
+        const int at = (P <= Q) ? packed_index<R>(P, Q) : -1;
+        if ((p & 3) == gbeta && at >= 0) dst[at] = v;
+      }
   };
 
   // stream indices of the next tile are prefetched behind the current
@@ -169,22 +202,15 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
     for (int t = 0; t < kSlots; ++t) io[t] = n_io[t];
     const float x = n_x;
     s_row[w][lane] = valid ? row : 0xFFFFFFFFu;
-    float v[4][4];
     if constexpr (KIND == kValu) {
       float d[R];
       if (tb + kWave < ce) load_idx(tb + kWave);
       contract_slots<R>(slots, a.A_L, iL, a.A_O, io, a.Gt, lrows, lane, d);
 #pragma unroll
-      for (int r = 0; r < 16; ++r)
-        tile[lane * kTP + r] = (r < R) ? (valid ? d[r] : 0.f) : (r == R ? x : 0.f);
-      wave_sync();
-#pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) v[mb][i] = tile[(mb * 16 + 4 * g + i) * kTP + q];
+      for (int r = 0; r < 4 * T; ++r)
+        tile[r * kDP + lane] = (r < R) ? (valid ? d[r] : 0.f) : (r == R ? x : 0.f);
     } else {
       s_iL[w][lane] = iL;
-      s_x[w][lane] = x;
       wave_sync();
       // every gather of the tile: the A_L row words of the 16 nonzeros each
       // MFMA block needs (per the lane group's plan), then this lane's outer rows
@@ -240,24 +266,27 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
         }
       }
       // A operands: kBf16 hi words (K16); kBf16R the plan's K32 words, plus
-      // hi words for the a_hi.G_lo MFMA when R > 10
-      s4 ahi[4];
-      s8 a32[4];
+      // hi words for the a_hi.G_lo MFMA when R > 10.  v_cvt_pk_bf16_f32
+      // rounds two floats to nearest even into one word pair.
+      u2 ahi[4];
+      u4 a32[4];
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
-        s4 lo0, hi1;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const short h = bf16_bits(f0[mb][t]);
-          ahi[mb][t] = h;
-          lo0[t] = bf16_bits(f0[mb][t] - bf16_value(h));
-          hi1[t] = bf16_bits(f1[mb][t]);
-        }
+        const uint32_t h01 = pk_bf16(f0[mb][0], f0[mb][1]);
+        const uint32_t h23 = pk_bf16(f0[mb][2], f0[mb][3]);
+        ahi[mb] = u2{h01, h23};
         if constexpr (KIND == kBf16R) {
-          s4 up = lo0;
-          if (plan == kPlanHH) up = hi1;
-          if (plan == kPlanR10) up = s4{lo0[0], lo0[1], ahi[mb][0], ahi[mb][1]};
-          a32[mb] = s8{ahi[mb][0], ahi[mb][1], ahi[mb][2], ahi[mb][3], up[0], up[1], up[2], up[3]};
+          const uint32_t l01 = pk_bf16(f0[mb][0] - bf16_low_f(h01), f0[mb][1] - bf16_high_f(h01));
+          const uint32_t l23 = pk_bf16(f0[mb][2] - bf16_low_f(h23), f0[mb][3] - bf16_high_f(h23));
+          uint32_t up0 = l01, up1 = l23;
+          if constexpr (bf16r_single_mfma(R)) {
+            if (plan == kPlanHH) {
+              up0 = pk_bf16(f1[mb][0], f1[mb][1]);
+              up1 = pk_bf16(f1[mb][2], f1[mb][3]);
+            }
+            if (plan == kPlanR10) up1 = h01;
+          }
+          a32[mb] = u4{h01, h23, up0, up1};
         }
       }
       wave_sync();
@@ -273,7 +302,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
           const s4 bh = t16[o * kWave + lane];
 #pragma unroll
           for (int mb = 0; mb < 4; ++mb)
-            u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[mb], bh, z, 0, 0, 0);
+            u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, ahi[mb]), bh, z,
+                                                              0, 0, 0);
         } else {
           const b8 b = __builtin_bit_cast(b8, t32[o * kWave + lane]);
 #pragma unroll
@@ -284,7 +314,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
             const s4 bl = t16lo[o * kWave + lane];
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb)
-              u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi[mb], bl, u[mb], 0, 0, 0);
+              u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, ahi[mb]), bl,
+                                                                u[mb], 0, 0, 0);
           }
         }
 #pragma unroll
@@ -302,12 +333,22 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
       } else {
         for (int o = 0; o < O; ++o) stage(o);
       }
+      // lane (q, g) holds delta[16mb + 4g + i][q]: columns to LDS, then x
+      // (zero for padding lanes) over column R
+      if (q < 4 * T) {
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          v[mb][i] = (q == R) ? s_x[w][mb * 16 + 4 * g + i] : dacc[mb][i];
+        for (int mb = 0; mb < 4; ++mb)
+          *reinterpret_cast<f4 *>(&tile[q * kDP + mb * 16 + 4 * g]) =
+              f4{dacc[mb][0], dacc[mb][1], dacc[mb][2], dacc[mb][3]};
+      }
+      tile[R * kDP + lane] = x;
     }
+    wave_sync();
+    float X[4][T]; // Gram operands of this lane (see above)
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int t = 0; t < T; ++t) X[mb][t] = tile[(4 * t + gx) * kDP + mb * 16 + 4 * gk + gbeta];
 
     // run boundaries inside the tile (uniform)
     const uint32_t myrow = s_row[w][lane];
@@ -320,7 +361,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
         cur = __builtin_amdgcn_readlane(myrow, s);
         run_begin = tb + s;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc64[r] = 0.0;
+        for (int p = 0; p < NP; ++p) acc[p] = 0.0;
       }
       const uint64_t above = (s >= 63) ? 0ull : (starts & (~0ull << (s + 1)));
       const int e_run = above ? (int)__builtin_ctzll(above) : cnt;
@@ -329,13 +370,20 @@ __global__ void __launch_bounds__(256) k_gram_mfma(GramArgs a, int wpb, int O) {
         const int lo = s > mb * 16 ? s : mb * 16;
         const int hi = e_run < mb * 16 + 16 ? e_run : mb * 16 + 16;
         if (lo >= hi) continue;
+        const int m = mb * 16 + 4 * gk + gbeta;
+        const bool in = m >= s && m < e_run;
+        double xa[T], xb[T];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          if (lo + ((i - lo) & 3) >= hi) continue; // no nonzero of this (mb, i) in the run
-          const int m = mb * 16 + 4 * g + i;
-          const double val = (m >= s && m < e_run) ? (double)v[mb][i] : 0.0;
-          acc64 = __builtin_amdgcn_mfma_f64_16x16x4f64(val, val, acc64, 0, 0, 0);
+        for (int t = 0; t < T; ++t) {
+          xb[t] = (double)X[mb][t];
+          xa[t] = (double)(in ? X[mb][t] : 0.f);
         }
+        int p = 0;
+#pragma unroll
+        for (int ta = 0; ta < T; ++ta)
+#pragma unroll
+          for (int tc = ta; tc < T; ++tc, ++p)
+            acc[p] = __builtin_amdgcn_mfma_f64_4x4x4f64(xa[ta], xb[tc], acc[p], 0, 0, 0);
       }
       s = e_run;
     }
