@@ -186,6 +186,7 @@ struct gt_ctx {
   // model
   bool initialized = false;
   DevBuf<float> A[GT_MAX_ORDER];
+  DevBuf<uint32_t> As[GT_MAX_ORDER]; // split BF16 copies of the factors (MFMA delta)
   DevBuf<float> Gt[GT_MAX_ORDER];
   DevBuf<uint16_t> Gb[GT_MAX_ORDER]; // BF16 core operand tables (MFMA delta)
   int O = 1;                         // R^(N-2)
@@ -486,6 +487,7 @@ GramArgs gram_args(gt_ctx *c, int n) {
   a.idx_row = c->st_idx[n][n].p;
   a.idx_L = c->st_idx[n][L].p;
   a.A_L = c->A[L].p;
+  a.As_L = c->As[L].p; // null unless a BF16 precision is selected
   int t = 0;
   for (int k = 0; k < c->N; ++k) {
     if (k == n || k == L) continue;
@@ -564,6 +566,14 @@ void run_gram(gt_ctx *c, int n) {
   launch_gram_fixup(c->R, a, c->s);
 }
 
+// Refreshes the split BF16 copy of factor n that the MFMA delta gathers
+// (BF16 precisions only).
+void refresh_split(gt_ctx *c, int n) {
+  if (c->o.precision == GT_PREC_FP32 || !gram_mfma_supported(c->R)) return;
+  c->As[n].alloc(c->rows[n] * kSplitWords);
+  launch_split_factor(c->R, c->A[n].p, c->rows[n], c->As[n].p, c->s);
+}
+
 // One mode update.  With fit_partials, the row solves also leave the
 // residual partials of the updated factor (valid as a fit only for the last
 // mode of an iteration); returns how many were written.
@@ -582,6 +592,7 @@ uint64_t mode_update(gt_ctx *c, int n, bool fit_partials = false) {
   sa.A = c->A[n].p;
   sa.fit_partials = fit_partials ? c->fit_part.p : nullptr;
   const uint64_t parts = launch_solve(c->R, sa, c->s);
+  refresh_split(c, n);
   GT_HIP(hipEventRecord(c->ev[2], c->s));
   GT_HIP(hipEventSynchronize(c->ev[2]));
   float g = 0.f, s = 0.f;
@@ -600,6 +611,7 @@ void upload_factor(gt_ctx *c, int n, const float *src) {
   for (uint64_t i = 0; i < c->dims[n]; ++i) std::memcpy(pad.data() + i * RS, src + i * R, R * 4);
   GT_HIP(hipMemcpyAsync(c->A[n].p, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice,
                         c->s));
+  refresh_split(c, n);
   GT_HIP(hipStreamSynchronize(c->s));
 }
 

@@ -17,21 +17,36 @@ __device__ inline __attribute__((always_inline)) void wave_sync() {
 
 typedef float gt_f4 __attribute__((ext_vector_type(4)));
 
-// row i of a factor (stride row_stride(R), zero padded) into registers with
-// 16-byte loads
+// row i of a factor (stride row_stride(R), zero padded) into registers:
+// 16-byte loads for whole groups of four columns, then 8- and 4-byte loads
+// for the rest, so no load writes a register the caller never reads (such a
+// register may be reused while the load is in flight, forcing a wait for
+// every outstanding load); the padding entries are set to zero.
+typedef float gt_f2 __attribute__((ext_vector_type(2)));
 template <int R>
 __device__ inline __attribute__((always_inline)) void load_row(const float *__restrict__ base,
                                                                uint32_t i,
                                                                float (&out)[row_stride(R)]) {
-  const gt_f4 *p = reinterpret_cast<const gt_f4 *>(base + (size_t)i * row_stride(R));
+  const float *row = base + (size_t)i * row_stride(R);
+  const gt_f4 *p = reinterpret_cast<const gt_f4 *>(row);
 #pragma unroll
-  for (int v = 0; v < row_stride(R) / 4; ++v) {
+  for (int v = 0; v < R / 4; ++v) {
     const gt_f4 t = p[v];
     out[4 * v + 0] = t[0];
     out[4 * v + 1] = t[1];
     out[4 * v + 2] = t[2];
     out[4 * v + 3] = t[3];
   }
+  constexpr int b = R / 4 * 4;
+  if constexpr (R % 4 >= 2) {
+    const gt_f2 t = *reinterpret_cast<const gt_f2 *>(row + b);
+    out[b] = t[0];
+    out[b + 1] = t[1];
+  }
+  if constexpr (R % 4 == 1) out[b] = row[b];
+  if constexpr (R % 4 == 3) out[b + 2] = row[b + 2];
+#pragma unroll
+  for (int j = R; j < row_stride(R); ++j) out[j] = 0.f;
 }
 
 // LDS floats per wave that contract_slots needs: the outer rows when there

@@ -138,6 +138,7 @@ struct GramArgs {
   const uint32_t *idx_O[kMaxOrder - 2];  // st.idx[outer t]
   const float *A_L;
   const float *A_O[kMaxOrder - 2];
+  const uint32_t *As_L;                  // split BF16 copy of A_L (16 words per row) or null
 };
 void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s);
 void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);
@@ -172,6 +173,10 @@ __host__ __device__ constexpr int bf16r_plan(int R, int g) {
 __host__ __device__ constexpr size_t gb_k16_words(int O) { return (size_t)O * kWave * 4; }
 __host__ __device__ constexpr size_t gb_k32_words(int O) { return (size_t)O * kWave * 8; }
 void launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
+// Split BF16 copy of a factor for the MFMA delta (R <= 15): 16 uint32 words
+// per row, the K=32 operand words of each lane group (bf16r_plan).
+constexpr int kSplitWords = 16;
+void launch_split_factor(int R, const float *A, uint64_t rows, uint32_t *As, hipStream_t s);
 
 struct SolveArgs {
   const double *gram;
