@@ -40,6 +40,30 @@ CONFIGS = {
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the
+    factors the same counters read for a kernel of known traffic), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"traffic_{workload}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or not t.get("traffic_bytes_per_launch"):
+        return None
+    return t["traffic_bytes_per_launch"]
+
+
+def kernel_name(precision, R):
+    """The delta + Gram kernel a precision runs at rank R."""
+    if R > 15:
+        return f"k_gram_fp32<{R}>"
+    kind = {"fp32": None, "bf16": 1, "bf16r": 2}[precision]
+    return f"k_gram_valu<{R}>" if kind is None else f"k_gram_bf16<{R}, {kind}>"
+
+
 def bytes_per_nnz(N, R, s_f):
     """Algorithmic HBM bytes per nonzero per mode update: N indices, the value,
     the row-grouping entry and the N-1 gathered factor rows."""
@@ -189,8 +213,9 @@ def main(argv=None):
                    "parallelism": "single GPU"},
         "fit": head["fit"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_gram_mfma", "kernel_ms": head["kernel_ms"],
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(a.config, kernel_name(a.precision, R)),
+                     "kernel": kernel_name(a.precision, R), "kernel_ms": head["kernel_ms"],
                      "bytes_per_nnz": bpn},
         "cpu_baseline": cpu,
     }

@@ -22,9 +22,14 @@ constexpr int kTile = 64;      // nonzeros per tile: one per lane
 // followed by C (R doubles).
 __host__ __device__ constexpr int gram_width(int R) { return R * (R + 1) / 2 + R; }
 
-// Factor rows are stored with a stride of R rounded up to 4 floats (padding
-// zero) so a row is read with 16-byte loads.
-__host__ __device__ constexpr int row_stride(int R) { return (R + 3) & ~3; }
+// Factor rows are stored with a stride of 4 or 8 floats (R <= 8), else R
+// rounded up to 16 floats, padding zero: a row is read with 16-byte loads and,
+// from a 64-byte-aligned start, a random row gather of up to 16 floats touches
+// one 64-byte memory segment (a 48-byte stride would straddle two for three
+// rows in four).
+__host__ __device__ constexpr int row_stride(int R) {
+  return R <= 4 ? 4 : (R <= 8 ? 8 : (R + 15) & ~15);
+}
 
 // Ranks the templated kernels are instantiated for.  A rank outside this list
 // is rejected with GT_ERR_UNSUPPORTED by the C ABI.
