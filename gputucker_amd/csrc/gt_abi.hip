@@ -207,6 +207,8 @@ struct gt_ctx {
   hipEvent_t ev_copy[2] = {}, ev_comp[2] = {};
   // timing
   hipEvent_t ev[8] = {};
+  hipEvent_t ev_mode[GT_MAX_ORDER][3] = {}; // per mode: start, Gram done, solve done
+  bool mode_pending[GT_MAX_ORDER] = {};     // recorded, not yet read into tm
   gt_timings tm{};
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending;
   std::vector<hipEvent_t> ev_pool;
@@ -222,6 +224,8 @@ struct gt_ctx {
     return e;
   }
   void release_host_streams() {
+    if (s) (void)hipStreamSynchronize(s); // queued copies may still read the buffers
+    if (s_copy) (void)hipStreamSynchronize(s_copy);
     for (int n = 0; n < GT_MAX_ORDER; ++n) {
       for (int k = 0; k < GT_MAX_ORDER; ++k)
         if (h_idx[n][k]) {
@@ -234,6 +238,20 @@ struct gt_ctx {
       }
     }
     streaming = false;
+  }
+  // Reads the event times of mode updates whose events have completed (the
+  // caller has synchronised the stream).
+  void collect_mode_timings() {
+    for (int n = 0; n < GT_MAX_ORDER; ++n) {
+      if (!mode_pending[n]) continue;
+      float g = 0.f, sv = 0.f;
+      GT_HIP(hipEventElapsedTime(&g, ev_mode[n][0], ev_mode[n][1]));
+      GT_HIP(hipEventElapsedTime(&sv, ev_mode[n][1], ev_mode[n][2]));
+      tm.ms_gram[n] = g;
+      tm.ms_solve[n] = sv;
+      tm.ms_mode[n] = g + sv;
+      mode_pending[n] = false;
+    }
   }
   void drain_profile() {
     for (auto &pr : prof_pending) {
@@ -257,6 +275,9 @@ struct gt_ctx {
       for (auto e : ev_pool) (void)hipEventDestroy(e);
       for (auto e : ev)
         if (e) (void)hipEventDestroy(e);
+      for (auto &em : ev_mode)
+        for (auto e : em)
+          if (e) (void)hipEventDestroy(e);
       release_host_streams();
       if (s_copy) {
         (void)hipStreamSynchronize(s_copy);
@@ -534,7 +555,9 @@ void launch_gram_window(gt_ctx *c, const GramArgs &a) {
 std::pair<uint64_t, uint64_t> stage_segment(gt_ctx *c, int n, uint64_t k) {
   const int b = (int)(k & 1);
   const uint64_t lo = k * c->seg_nnz, hi = std::min(lo + c->seg_nnz, c->nnz), len = hi - lo;
-  if (k >= 2) GT_HIP(hipStreamWaitEvent(c->s_copy, c->ev_comp[b], 0)); // buffer free again
+  // the last kernel that read buffer b (this pass or, since mode updates are
+  // queued without a host wait, the previous one) must be done with it
+  GT_HIP(hipStreamWaitEvent(c->s_copy, c->ev_comp[b], 0));
   for (int m = 0; m < c->N; ++m)
     GT_HIP(hipMemcpyAsync(c->stage_idx[b][m].p, c->h_idx[n][m] + lo, len * sizeof(uint32_t),
                           hipMemcpyHostToDevice, c->s_copy));
@@ -574,16 +597,17 @@ void refresh_split(gt_ctx *c, int n) {
   launch_split_factor(c->R, c->A[n].p, c->rows[n], c->As[n].p, c->s);
 }
 
-// One mode update.  With fit_partials, the row solves also leave the
+// One mode update, queued on the context's stream (sync: wait for it and
+// read its timings).  With fit_partials, the row solves also leave the
 // residual partials of the updated factor (valid as a fit only for the last
 // mode of an iteration); returns how many were written.
-uint64_t mode_update(gt_ctx *c, int n, bool fit_partials = false) {
+uint64_t mode_update(gt_ctx *c, int n, bool fit_partials, bool sync) {
   require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
   require(n >= 0 && n < c->N, GT_ERR_ARG, "mode out of range");
   if (c->prof_pending.size() > 64) c->drain_profile();
-  GT_HIP(hipEventRecord(c->ev[0], c->s));
+  GT_HIP(hipEventRecord(c->ev_mode[n][0], c->s));
   run_gram(c, n);
-  GT_HIP(hipEventRecord(c->ev[1], c->s));
+  GT_HIP(hipEventRecord(c->ev_mode[n][1], c->s));
   SolveArgs sa{};
   sa.gram = c->gram.p;
   sa.row_ptr = c->row_ptr[n].p;
@@ -593,14 +617,12 @@ uint64_t mode_update(gt_ctx *c, int n, bool fit_partials = false) {
   sa.fit_partials = fit_partials ? c->fit_part.p : nullptr;
   const uint64_t parts = launch_solve(c->R, sa, c->s);
   refresh_split(c, n);
-  GT_HIP(hipEventRecord(c->ev[2], c->s));
-  GT_HIP(hipEventSynchronize(c->ev[2]));
-  float g = 0.f, s = 0.f;
-  GT_HIP(hipEventElapsedTime(&g, c->ev[0], c->ev[1]));
-  GT_HIP(hipEventElapsedTime(&s, c->ev[1], c->ev[2]));
-  c->tm.ms_gram[n] = g;
-  c->tm.ms_solve[n] = s;
-  c->tm.ms_mode[n] = g + s;
+  GT_HIP(hipEventRecord(c->ev_mode[n][2], c->s));
+  c->mode_pending[n] = true;
+  if (sync) {
+    GT_HIP(hipEventSynchronize(c->ev_mode[n][2]));
+    c->collect_mode_timings();
+  }
   return parts;
 }
 
@@ -686,14 +708,16 @@ double compute_fit(gt_ctx *c) {
 // sum_e (x_e - xhat_e)^2 = |x|^2 + sum_i (a_i^T G_i a_i - 2 a_i^T C_i), so no
 // reconstruction pass over the nonzeros is needed.
 double iteration(gt_ctx *c) {
-  for (int n = 0; n + 1 < c->N; ++n) mode_update(c, n);
-  const uint64_t parts = mode_update(c, c->N - 1, true);
+  // the N updates are queued back to back; the host waits once, for the fit
+  for (int n = 0; n + 1 < c->N; ++n) mode_update(c, n, false, false);
+  const uint64_t parts = mode_update(c, c->N - 1, true, false);
   GT_HIP(hipEventRecord(c->ev[3], c->s));
   reduce_f64(c->fit_part.p, (int)parts, c->scalar.p, c->s);
   double d = 0.0;
   GT_HIP(hipMemcpyAsync(&d, c->scalar.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
   GT_HIP(hipEventRecord(c->ev[4], c->s));
   GT_HIP(hipStreamSynchronize(c->s));
+  c->collect_mode_timings();
   float ms = 0.f;
   GT_HIP(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
   c->tm.ms_fit = ms;
@@ -956,6 +980,8 @@ int gt_create(gt_ctx **out, const gt_opts *opts) {
     GT_HIP(hipSetDevice(opts->device));
     GT_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto &e : c->ev) GT_HIP(hipEventCreate(&e));
+    for (auto &em : c->ev_mode)
+      for (auto &e : em) GT_HIP(hipEventCreate(&e));
   } catch (const GtError &e) {
     g_create_error = e.what();
     return e.code;
@@ -1144,7 +1170,7 @@ int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core) {
 }
 
 int gt_mode_update(gt_ctx *ctx, int mode) {
-  return guarded(ctx, [&] { mode_update(ctx, mode); });
+  return guarded(ctx, [&] { mode_update(ctx, mode, false, true); });
 }
 
 int gt_fit(gt_ctx *ctx, double *fit) {
@@ -1262,6 +1288,8 @@ int gt_get_gram(gt_ctx *ctx, int mode, double *B, double *C) {
 int gt_get_timings(gt_ctx *ctx, gt_timings *t) {
   return guarded(ctx, [&] {
     require(t != nullptr, GT_ERR_ARG, "null");
+    GT_HIP(hipStreamSynchronize(ctx->s));
+    ctx->collect_mode_timings();
     ctx->drain_profile();
     *t = ctx->tm;
   });

@@ -132,10 +132,10 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
 // (the row starts at or before the chunk's first position), else in
 // bpart1[c].
 //
-// Level 1, one wave per piece of 64 chunks: for each row whose run goes on
+// Level 1, one workgroup per piece of 64 chunks: for each row whose run goes on
 // from chunk c into later chunks of the same piece, the partials of those
 // later chunks (always bpart0) are added into c's slot.
-// Level 2, one wave per piece: for each row that starts in chunk c and
+// Level 2, one workgroup per piece: for each row that starts in chunk c and
 // continues past it, c's slot (now the row's sum over its first piece) plus
 // bpart0 of the first chunk of each later piece the row covers (the row's
 // sum over that piece) is the row's Gram entry.  Every sum runs in a fixed
@@ -156,18 +156,28 @@ __device__ inline double sum_bpart0(const double *bp, uint64_t first, uint64_t l
   return (s0 + s1) + (s2 + s3);
 }
 
+// Each fix-up workgroup has kFixWaves waves.  All waves find the same row
+// segments from the chunk tails; independent segments are dealt to the
+// waves in turn, and a row spread over many pieces is summed by all waves
+// together (wave w takes pieces w, w + kFixWaves, ...; wave 0 adds the
+// per-wave sums in wave order).
+constexpr int kFixWaves = 4;
+constexpr uint64_t kLongRow = 8; // pieces above which all waves share a row
+
 template <int R>
-__global__ void __launch_bounds__(kWave) k_gram_fix_pieces(GramArgs a) {
+__global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_pieces(GramArgs a) {
   constexpr int W = gram_width(R);
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
   const uint32_t none = 0xFFFFFFFFu;
   const uint32_t t = c < a.nchunks ? a.chunk_tail[c] : none;               // c -> c + 1
   const uint32_t pv = (c > 0 && c < a.nchunks) ? a.chunk_tail[c - 1] : none; // c - 1 -> c
   uint64_t starts = __ballot(t != none && (lane == 0 || pv != t));
+  int k = 0;
   while (starts) {
     const int j = __builtin_ctzll(starts);
     starts &= starts - 1;
+    if ((k++ % kFixWaves) != w) continue; // this segment belongs to another wave
     const uint32_t r = __builtin_amdgcn_readlane(t, j);
     const uint64_t bal = __ballot(pv == r); // chunks the row runs into from the one before
     const uint64_t into = j >= kPiece - 1 ? 0ull : bal >> (j + 1);
@@ -181,14 +191,16 @@ __global__ void __launch_bounds__(kWave) k_gram_fix_pieces(GramArgs a) {
 }
 
 template <int R>
-__global__ void __launch_bounds__(kWave) k_gram_fix_rows(GramArgs a) {
+__global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_rows(GramArgs a) {
   constexpr int W = gram_width(R);
-  const int lane = threadIdx.x;
+  __shared__ double red[kFixWaves][W];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
   const uint32_t none = 0xFFFFFFFFu;
   const uint32_t t = c < a.nchunks ? a.chunk_tail[c] : none;
   const bool begins = t != none && a.row_ptr[t] >= c * a.chunk; // the row starts in chunk c
   uint64_t mask = __ballot(begins);
+  int k = 0;
   while (mask) {
     const int j = __builtin_ctzll(mask);
     mask &= mask - 1;
@@ -196,12 +208,26 @@ __global__ void __launch_bounds__(kWave) k_gram_fix_rows(GramArgs a) {
     const uint64_t cj = (uint64_t)blockIdx.x * kPiece + j;
     const uint64_t rb = a.row_ptr[r];
     const uint64_t last = (a.row_ptr[r + 1] - 1) / a.chunk;
-    const double *first = (rb == cj * a.chunk ? a.bpart0 : a.bpart1) + cj * W;
     const uint64_t p0 = cj / kPiece, pl = last / kPiece;
-    for (int q = lane; q < W; q += kWave) {
-      double s = first[q];
-      if (pl > p0) s += sum_bpart0(a.bpart0, (p0 + 1) * kPiece, pl * kPiece, kPiece, W, q);
-      a.gram[(size_t)r * W + q] = s;
+    const double *first = (rb == cj * a.chunk ? a.bpart0 : a.bpart1) + cj * W;
+    if (pl - p0 > kLongRow) { // the same decision in every wave
+      for (int q = lane; q < W; q += kWave)
+        red[w][q] =
+            sum_bpart0(a.bpart0, (p0 + 1 + w) * kPiece, pl * kPiece, kPiece * kFixWaves, W, q);
+      __syncthreads();
+      if (w == 0) {
+        for (int q = lane; q < W; q += kWave)
+          a.gram[(size_t)r * W + q] =
+              first[q] + ((red[0][q] + red[1][q]) + (red[2][q] + red[3][q]));
+      }
+      __syncthreads();
+    } else {
+      if ((k++ % kFixWaves) != w) continue;
+      for (int q = lane; q < W; q += kWave) {
+        double s = first[q];
+        if (pl > p0) s += sum_bpart0(a.bpart0, (p0 + 1) * kPiece, pl * kPiece, kPiece, W, q);
+        a.gram[(size_t)r * W + q] = s;
+      }
     }
   }
 }
@@ -279,8 +305,8 @@ template <int R> void gram_fp32_impl(const GramArgs &a, hipStream_t s) {
 template <int R> void gram_fixup_impl(const GramArgs &a, hipStream_t s) {
   if (a.nchunks <= 1) return;
   const unsigned grid = (unsigned)((a.nchunks + kPiece - 1) / kPiece);
-  hipLaunchKernelGGL(k_gram_fix_pieces<R>, dim3(grid), dim3(kWave), 0, s, a);
-  hipLaunchKernelGGL(k_gram_fix_rows<R>, dim3(grid), dim3(kWave), 0, s, a);
+  hipLaunchKernelGGL(k_gram_fix_pieces<R>, dim3(grid), dim3(kWave * kFixWaves), 0, s, a);
+  hipLaunchKernelGGL(k_gram_fix_rows<R>, dim3(grid), dim3(kWave * kFixWaves), 0, s, a);
 }
 
 template <int R> int coo_impl(const CooArgs &a, hipStream_t s) {

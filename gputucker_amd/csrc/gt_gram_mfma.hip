@@ -172,11 +172,12 @@ template <int R> struct ChunkGram {
         if (lo >= hi) continue;
         const int m = mb * 16 + 4 * gk + gbeta;
         const bool in = m >= s && m < e_run;
+        const bool whole = s <= mb * 16 && e_run >= mb * 16 + 16; // the run covers the group
         double xa[T], xb[T];
 #pragma unroll
         for (int t = 0; t < T; ++t) {
           xb[t] = (double)X[mb][t];
-          xa[t] = (double)(in ? X[mb][t] : 0.f);
+          xa[t] = whole ? xb[t] : (double)(in ? X[mb][t] : 0.f);
         }
         int p = 0;
 #pragma unroll
@@ -361,8 +362,10 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
     if (slots == 0) {
       souter[lane] = valid ? 1.f : 0.f;
     } else if (slots == 1) {
+      // padding lanes gathered in-bounds rows; their deltas never enter the
+      // Gram (the runs end at the tile's last valid nonzero)
 #pragma unroll
-      for (int o = 0; o < R; ++o) souter[o * kWave + lane] = valid ? aO[o] : 0.f;
+      for (int o = 0; o < R; ++o) souter[o * kWave + lane] = aO[o];
     } else {
 #pragma unroll
       for (int t = 0; t < kSlots; ++t) {

@@ -260,3 +260,61 @@ def test_iteration_fit_matches_reconstruction(prec, tol, N, R, part):
         f_iter = e.iteration()
         f_rec = e.fit()
         assert abs(f_iter - f_rec) <= tol * abs(f_rec), (f_iter, f_rec)
+
+
+def _one_iteration_vs_oracle(dims, nnz, modes, values, R, part, prec, seed, tol, **kw):
+    from gputucker_amd import synth
+    idx, vals, dims = synth.generate(dims, nnz, modes, values, seed=seed)
+    A, G = synth.init_factors(dims, R, seed=seed)
+    lam = gt.default_lambda()
+    e = make(idx, vals, dims, R, part, precision=prec, max_iters=1, tol=0.0, **kw)
+    e.set_init(A, G)
+    fits = e.run()
+    t = oracle.Tensor(idx, vals.astype(np.float64), dims, part, R)
+    Ap = [oracle.pad_factor(a.astype(np.float64), t.rows(k)) for k, a in enumerate(A)]
+    f64, _ = oracle.als(t, Ap, G.astype(np.float64), lam, 1, 0.0, "f64")
+    assert np.isfinite(fits).all()
+    assert abs(fits[0] - f64[0]) / abs(f64[0]) < tol, (fits, f64)
+    return e
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16r", 1e-4)])
+def test_order4_rank8_sample(prec, tol):
+    """Shape of the 4-order workload (three Zipf(1.2) modes, one small uniform
+    mode, lognormal values, R = 8), at 4e4 nonzeros: one ALS iteration vs the
+    FP64 oracle."""
+    _one_iteration_vs_oracle([20_000, 20_000, 20_000, 100], 40_000,
+                             [("zipf", 1.2)] * 3 + ["uniform"], "lognormal", 8, [2, 2, 1, 1],
+                             prec, 5, tol)
+
+
+def test_rank32_sample():
+    """Shape of the rank-32 workload (Zipf(1.5), 5e5 x 5e5 x 5e4), at 1.5e4
+    nonzeros: the VALU delta + FP64 Gram for R > 15 and the wave-per-row solve."""
+    _one_iteration_vs_oracle([50_000, 50_000, 5_000], 15_000, [("zipf", 1.5)] * 3, "uniform",
+                             32, [1, 1, 1], "fp32", 6, 1e-5)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16r"])
+def test_order5_out_of_core_sample(prec):
+    """Shape of the 5-order rank-5 workload that runs under a device budget:
+    streams staged from pinned host memory, factors bit-identical to the
+    in-core run, fit against the FP64 oracle."""
+    from gputucker_amd import synth
+    dims0 = [500, 500, 500, 500, 20]
+    idx, vals, dims = synth.generate(dims0, 30_000, None, "uniform", seed=9)
+    A, G = synth.init_factors(dims, 5, seed=9)
+    kw = dict(precision=prec, max_iters=1, tol=0.0, chunk_tiles=1)
+    ref = make(idx, vals, dims, 5, [1, 1, 1, 1, 1], **kw)
+    ref.set_init(A, G)
+    oc = make(idx, vals, dims, 5, [1, 1, 1, 1, 1], device_budget=2_000_000, **kw)
+    oc.set_init(A, G)
+    assert oc.info()["out_of_core"] == 1 and ref.info()["out_of_core"] == 0
+    f_ref, f_oc = ref.run(), oc.run()
+    for m in range(5):
+        assert np.array_equal(ref.get_factor(m), oc.get_factor(m)), m
+    assert np.allclose(f_ref, f_oc, rtol=0, atol=1e-12)
+    t = oracle.Tensor(idx, vals.astype(np.float64), dims, [1, 1, 1, 1, 1], 5)
+    Ap = [oracle.pad_factor(a.astype(np.float64), t.rows(k)) for k, a in enumerate(A)]
+    f64, _ = oracle.als(t, Ap, G.astype(np.float64), gt.default_lambda(), 1, 0.0, "f64")
+    assert abs(f_oc[0] - f64[0]) / abs(f64[0]) < (1e-5 if prec == "fp32" else 1e-4)
