@@ -173,6 +173,8 @@ template <int R> struct ChunkGram {
         const int m = mb * 16 + 4 * gk + gbeta;
         const bool in = m >= s && m < e_run;
         const bool whole = s <= mb * 16 && e_run >= mb * 16 + 16; // the run covers the group
+        // (selects, not a branch: MFMA ignores EXEC, so no MFMA may sit under
+        // a condition the compiler could turn into an EXEC mask)
         double xa[T], xb[T];
 #pragma unroll
         for (int t = 0; t < T; ++t) {
