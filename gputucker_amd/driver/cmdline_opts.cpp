@@ -34,7 +34,10 @@ void CommandLineOptions::PrintHelp() const {
                "  --device-budget bytes         Memory the planner may assume (default: free)\n"
                "  --precision fp32|bf16|bf16r   Delta contraction precision (default fp32)\n"
                "  --device arg (=0)             HIP device ordinal\n"
-               "  --output prefix               Write factors and core as text files\n"
+               "  --output prefix               Write factors, core and fit history as text files\n"
+               "  --resume prefix               Start from the factors and core of an --output run\n"
+               "  --fused-fit                   Take each iteration's fit from the row solves\n"
+               "  --cache                       Read/write a binary copy of the input (<input>.gtcoo)\n"
                "  --parse-only                  Parse the input, print its summary, exit (no GPU)\n";
 }
 
@@ -145,6 +148,13 @@ CommandLineOptions::ReturnStatus CommandLineOptions::Parse(int argc, char *argv[
     } else if (a == "--output") {
       if (!need(v)) return OPTS_FAILURE;
       _output = v;
+    } else if (a == "--resume") {
+      if (!need(v)) return OPTS_FAILURE;
+      _resume = v;
+    } else if (a == "--fused-fit") {
+      _fused_fit = true;
+    } else if (a == "--cache") {
+      _cache = true;
     } else {
       std::cout << "[ERROR] Parsing error : unrecognised option '" << argv[i] << "'" << std::endl;
       return OPTS_FAILURE;

@@ -49,6 +49,9 @@ public:
   int get_device() const { return _device; }
   const std::string &get_output_prefix() const { return _output; }
   bool get_parse_only() const { return _parse_only; }
+  bool get_cache() const { return _cache; }
+  const std::string &get_resume_prefix() const { return _resume; }
+  bool get_fused_fit() const { return _fused_fit; }
   void PrintHelp() const;
 
 private:
@@ -66,6 +69,9 @@ private:
   int _device = 0;
   std::string _output;
   bool _parse_only = false;
+  bool _cache = false;       // read/write a binary COO copy beside the input
+  std::string _resume;       // start from factors and core written by --output
+  bool _fused_fit = false;   // fit from the row solves (gt_iteration)
 };
 
 template <typename IndexType, typename ValueType> class Block {
@@ -127,6 +133,13 @@ using tensor_t = Tensor<block_t>;
 // Reads a base-1 tab/space separated coordinate file (README.md:78-97).
 // Throws std::runtime_error like tensor_manager.tpp:21-25,166-168.
 template <typename TensorType> bool ParseFromFile(const std::string &file_name, TensorType **tensor);
+
+// Binary COO cache of a parsed input: `<input>.gtcoo` holds the order, the
+// nonzero count, the dims, the 0-based indices and the values as the parser
+// left them.  LoadCache returns false when the cache is missing, older than
+// the input or of another order; SaveCache writes it.
+bool LoadCache(const std::string &input, tensor_t *tensor);
+void SaveCache(const std::string &input, const tensor_t *tensor);
 
 // Partitions on the GPU (engine already created) and, when with_host_blocks,
 // copies the blocks and their per-mode row grouping back into tensor->blocks.

@@ -17,7 +17,10 @@ int main(int argc, char *argv[]) {
   printf("Values are float type.\n");
   try {
     tensor_t *input = new tensor_t((unsigned short)options.get_order());
-    ParseFromFile<tensor_t>(options.get_input_path(), &input);
+    if (!options.get_cache() || !LoadCache(options.get_input_path(), input)) {
+      ParseFromFile<tensor_t>(options.get_input_path(), &input);
+      if (options.get_cache()) SaveCache(options.get_input_path(), input);
+    }
     if (options.get_parse_only()) { // summary for scripts and tests
       double ss = 0.0;
       for (float v : input->coo_values) ss += (double)v * v;

@@ -222,6 +222,58 @@ template <> bool ParseFromFile<tensor_t>(const std::string &file_name, tensor_t 
   return true;
 }
 
+// ------------------------------------------------------------ COO cache
+namespace {
+constexpr char kCacheMagic[8] = {'G', 'T', 'C', 'O', 'O', '1', 0, 0};
+std::string cache_path(const std::string &input) { return input + ".gtcoo"; }
+} // namespace
+
+bool LoadCache(const std::string &input, tensor_t *t) {
+  const std::string path = cache_path(input);
+  struct stat si, sc;
+  if (stat(input.c_str(), &si) != 0 || stat(path.c_str(), &sc) != 0) return false;
+  if (sc.st_mtime < si.st_mtime) return false; // the input changed since
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return false;
+  char magic[8];
+  uint32_t order = 0;
+  uint64_t nnz = 0;
+  bool ok = fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kCacheMagic, 8) == 0 &&
+            fread(&order, sizeof order, 1, f) == 1 && order == t->order &&
+            fread(&nnz, sizeof nnz, 1, f) == 1 &&
+            fread(t->dims, sizeof(uint32_t), order, f) == order;
+  for (uint32_t a = 0; ok && a < order; ++a) {
+    t->coo_indices[a].resize(nnz);
+    ok = fread(t->coo_indices[a].data(), sizeof(uint32_t), nnz, f) == nnz;
+  }
+  if (ok) {
+    t->coo_values.resize(nnz);
+    ok = fread(t->coo_values.data(), sizeof(float), nnz, f) == nnz;
+  }
+  fclose(f);
+  if (!ok) return false;
+  t->nnz_count = nnz;
+  printf("Input Tensor read from the binary cache %s\n", path.c_str());
+  return true;
+}
+
+void SaveCache(const std::string &input, const tensor_t *t) {
+  const std::string path = cache_path(input), tmp = path + ".tmp";
+  FILE *f = fopen(tmp.c_str(), "wb");
+  if (!f) return; // a cache is an optimisation: no cache, no error
+  const uint32_t order = t->order;
+  const uint64_t nnz = t->nnz_count;
+  bool ok = fwrite(kCacheMagic, 1, 8, f) == 8 && fwrite(&order, sizeof order, 1, f) == 1 &&
+            fwrite(&nnz, sizeof nnz, 1, f) == 1 &&
+            fwrite(t->dims, sizeof(uint32_t), order, f) == order;
+  for (uint32_t a = 0; ok && a < order; ++a)
+    ok = fwrite(t->coo_indices[a].data(), sizeof(uint32_t), nnz, f) == nnz;
+  if (ok) ok = fwrite(t->coo_values.data(), sizeof(float), nnz, f) == nnz;
+  ok = (fclose(f) == 0) && ok;
+  if (ok) ok = rename(tmp.c_str(), path.c_str()) == 0;
+  if (!ok) unlink(tmp.c_str());
+}
+
 // ------------------------------------------------------- CreateTensorBlocks
 static void check(gt_ctx *e, int rc, const char *what) {
   if (rc != GT_OK)

@@ -6,6 +6,8 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <iomanip>
+#include <sstream>
 #include <stdexcept>
 #include <vector>
 
@@ -19,11 +21,27 @@ void check(gt_ctx *e, int rc, const char *what) {
   if (rc != GT_OK) throw std::runtime_error(std::string(what) + " failed: " + gt_last_error(e));
 }
 
+// Text matrices, 9 significant digits so every float reads back exactly.
 void write_matrix(const std::string &path, const float *m, uint64_t rows, int cols) {
   std::ofstream f(path);
+  f << std::setprecision(9);
   for (uint64_t r = 0; r < rows; ++r) {
     for (int c = 0; c < cols; ++c) f << m[r * cols + c] << (c + 1 < cols ? '\t' : '\n');
   }
+  if (!f) throw std::runtime_error("[ERROR] cannot write \"" + path + "\"");
+}
+
+// Reads a rows x cols matrix written by write_matrix (whitespace separated).
+void read_matrix(const std::string &path, float *m, uint64_t rows, int cols) {
+  std::ifstream f(path);
+  if (!f) throw std::runtime_error("[ERROR] cannot open \"" + path + "\" to resume");
+  for (uint64_t i = 0; i < rows * (uint64_t)cols; ++i) {
+    if (!(f >> m[i]))
+      throw std::runtime_error("[ERROR] \"" + path + "\" holds fewer than " +
+                               std::to_string(rows) + " x " + std::to_string(cols) + " values");
+  }
+  float extra;
+  if (f >> extra) throw std::runtime_error("[ERROR] \"" + path + "\" holds more values than expected");
 }
 } // namespace
 
@@ -53,39 +71,68 @@ void TuckerDecomposition(tensor_t *t, const CommandLineOptions &o) {
   printf("\t... Make the factor matrices\n");
   std::vector<std::vector<float>> A(N);
   std::vector<const float *> Ap(N);
-  for (int a = 0; a < N; ++a) {
-    printf("\t\t... Fill the factor matrix [%d]\n", a);
-    A[a].resize((uint64_t)t->dims[a] * R);
-    for (uint64_t i = 0; i < A[a].size(); ++i) A[a][i] = (float)frand<double>(0, 1);
-    Ap[a] = A[a].data();
-  }
-  printf("\t... Make the core tensor\n"); // tucker.cuh:118-119, linear order
   uint64_t core_n = 1;
   for (int a = 0; a < N; ++a) core_n *= (uint64_t)R;
   std::vector<float> core(core_n);
-  for (uint64_t i = 0; i < core_n; ++i) core[i] = (float)frand<double>(0, 1);
+  const std::string &resume = o.get_resume_prefix();
+  for (int a = 0; a < N; ++a) {
+    A[a].resize((uint64_t)t->dims[a] * R);
+    if (resume.empty()) {
+      printf("\t\t... Fill the factor matrix [%d]\n", a);
+      for (uint64_t i = 0; i < A[a].size(); ++i) A[a][i] = (float)frand<double>(0, 1);
+    } else {
+      printf("\t\t... Read the factor matrix [%d] from %s_factor%d.txt\n", a, resume.c_str(), a);
+      read_matrix(resume + "_factor" + std::to_string(a) + ".txt", A[a].data(), t->dims[a], R);
+    }
+    Ap[a] = A[a].data();
+  }
+  if (resume.empty()) {
+    printf("\t... Make the core tensor\n"); // tucker.cuh:118-119, linear order
+    for (uint64_t i = 0; i < core_n; ++i) core[i] = (float)frand<double>(0, 1);
+  } else {
+    printf("\t... Read the core tensor from %s_core.txt\n", resume.c_str());
+    read_matrix(resume + "_core.txt", core.data(), core_n / R, R);
+  }
   check(t->engine, gt_set_init(t->engine, Ap.data(), core.data()), "gt_set_init");
 
   int iter = 0;
   double p_fit = -1, fit = -1;
+  std::ostringstream history; // iteration, fit, seconds
+  history << std::setprecision(17);
   for (;;) { // tucker.cuh:166-187
     double itertime = omp_get_wtime();
     double steptime = itertime;
-    for (int n = 0; n < N; ++n) {
-      printf("[ Update factor matrix %d ]\n", n);
-      check(t->engine, gt_mode_update(t->engine, n), "gt_mode_update");
+    if (o.get_fused_fit()) {
+      // the N updates back to back, the fit from the last mode's row solves
+      check(t->engine, gt_iteration(t->engine, &fit), "gt_iteration");
       gt_timings tm;
       check(t->engine, gt_get_timings(t->engine, &tm), "gt_get_timings");
-      printf("\t- Elapsed time for Computing Delta and B, C (fused): %lf\n", tm.ms_gram[n] / 1e3);
-      printf("\t- row-wise update TIME : %lf\n", tm.ms_solve[n] / 1e3);
+      for (int n = 0; n < N; ++n) {
+        printf("[ Update factor matrix %d ]\n", n);
+        printf("\t- Elapsed time for Computing Delta and B, C (fused): %lf\n", tm.ms_gram[n] / 1e3);
+        printf("\t- row-wise update TIME : %lf\n", tm.ms_solve[n] / 1e3);
+      }
+      printf("Factor Time : %lf\n", omp_get_wtime() - steptime);
+      printf("Recon Time : %lf\n\n", tm.ms_fit / 1e3);
+    } else {
+      for (int n = 0; n < N; ++n) {
+        printf("[ Update factor matrix %d ]\n", n);
+        check(t->engine, gt_mode_update(t->engine, n), "gt_mode_update");
+        gt_timings tm;
+        check(t->engine, gt_get_timings(t->engine, &tm), "gt_get_timings");
+        printf("\t- Elapsed time for Computing Delta and B, C (fused): %lf\n", tm.ms_gram[n] / 1e3);
+        printf("\t- row-wise update TIME : %lf\n", tm.ms_solve[n] / 1e3);
+      }
+      printf("Factor Time : %lf\n", omp_get_wtime() - steptime);
+      steptime = omp_get_wtime();
+      check(t->engine, gt_fit(t->engine, &fit), "gt_fit");
+      printf("Recon Time : %lf\n\n", omp_get_wtime() - steptime);
     }
-    printf("Factor Time : %lf\n", omp_get_wtime() - steptime);
-    steptime = omp_get_wtime();
-    check(t->engine, gt_fit(t->engine, &fit), "gt_fit");
-    printf("Recon Time : %lf\n\n", omp_get_wtime() - steptime);
     ++iter;
+    const double elapsed = omp_get_wtime() - itertime;
+    history << iter << '\t' << fit << '\t' << elapsed << '\n';
     printf("iter %d\t Fit: %g\n", iter, fit);
-    printf("iter%d :      Fit : %lf\tElapsed Time : %lf\n\n", iter, fit, omp_get_wtime() - itertime);
+    printf("iter%d :      Fit : %lf\tElapsed Time : %lf\n\n", iter, fit, elapsed);
     if (iter >= o.get_iterations() || (p_fit != -1 && std::fabs(p_fit - fit) <= o.get_tol())) break;
     p_fit = fit;
   }
@@ -96,6 +143,7 @@ void TuckerDecomposition(tensor_t *t, const CommandLineOptions &o) {
                    t->dims[a], R);
     }
     write_matrix(o.get_output_prefix() + "_core.txt", core.data(), core_n / R, R);
+    std::ofstream(o.get_output_prefix() + "_fit.txt") << history.str();
   }
   printf("DONE\n");
 }

@@ -73,3 +73,35 @@ def test_option_errors(tmp_path):
     rc, out = run("-h")
     assert rc == 0 and "--input" in out and "--rank" in out and "--gpus" in out
     assert run("-i", README_TNS, "-o", "3", "--partition", "1,2")[0] != 0
+
+
+def test_binary_cache_round_trip(tmp_path):
+    p = tmp_path / "t.tns"
+    p.write_text(open(README_TNS).read())
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only", "--cache")
+    assert rc == 0 and "binary cache" not in out
+    cache = tmp_path / "t.tns.gtcoo"
+    assert cache.exists()
+    first = summary(out)
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only", "--cache")
+    assert rc == 0 and "binary cache" in out
+    assert summary(out) == first
+    # an input newer than its cache is parsed again
+    os.utime(cache, (1, 1))
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only", "--cache")
+    assert rc == 0 and "binary cache" not in out and summary(out) == first
+
+
+def test_cache_of_another_order_is_ignored(tmp_path):
+    p = tmp_path / "t.tns"
+    p.write_text("1 1 1 1 2.0\n2 1 1 1 3.0\n")
+    rc, out = run("-i", str(p), "-o", "4", "--parse-only", "--cache")
+    assert rc == 0
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only", "--cache")
+    assert rc == 0 and "binary cache" not in out
+    assert summary(out)[:3] == (3, 2, [2, 1, 1])
+
+
+def test_new_flags_are_accepted():
+    rc, out = run("-i", README_TNS, "-o", "3", "--parse-only", "--fused-fit", "--resume", "x")
+    assert rc == 0, out
