@@ -49,23 +49,8 @@ void TuckerDecomposition(tensor_t *t, const CommandLineOptions &o) {
   const int N = t->order, R = o.get_rank();
   if (o.get_gpu_count() != 1)
     printf("[WARNING] this engine runs on one GPU; using device %d\n", o.get_device());
-  gt_opts opts;
-  gt_default_opts(&opts);
-  opts.device = o.get_device();
-  opts.order = N;
-  opts.rank = R;
-  opts.lambda = o.get_lambda();
-  opts.max_iters = o.get_iterations();
-  opts.tol = o.get_tol();
-  opts.precision = o.get_precision();
-  opts.device_budget = o.get_device_budget();
-  for (size_t a = 0; a < o.get_partition().size(); ++a) opts.partition[a] = o.get_partition()[a];
-  printf("\t... Initialize the GPU engine\n");
-  if (gt_create(&t->engine, &opts) != GT_OK)
-    throw std::runtime_error(std::string("gt_create failed: ") + gt_last_error(nullptr));
-  CreateTensorBlocks(t, false);
-  t->ToString();
-
+  // The factors and the core are drawn before the GPU runtime starts: its
+  // threads may call rand() too, which would shift the sequence.
   // tucker.cuh:84-97: factors by axis, then part, then row, then column; the
   // rows of one axis are visited in global order whatever the partition.
   printf("\t... Make the factor matrices\n");
@@ -93,6 +78,23 @@ void TuckerDecomposition(tensor_t *t, const CommandLineOptions &o) {
     printf("\t... Read the core tensor from %s_core.txt\n", resume.c_str());
     read_matrix(resume + "_core.txt", core.data(), core_n / R, R);
   }
+  gt_opts opts;
+  gt_default_opts(&opts);
+  opts.device = o.get_device();
+  opts.order = N;
+  opts.rank = R;
+  opts.lambda = o.get_lambda();
+  opts.max_iters = o.get_iterations();
+  opts.tol = o.get_tol();
+  opts.precision = o.get_precision();
+  opts.device_budget = o.get_device_budget();
+  for (size_t a = 0; a < o.get_partition().size(); ++a) opts.partition[a] = o.get_partition()[a];
+  printf("\t... Initialize the GPU engine\n");
+  if (gt_create(&t->engine, &opts) != GT_OK)
+    throw std::runtime_error(std::string("gt_create failed: ") + gt_last_error(nullptr));
+  CreateTensorBlocks(t, false);
+  t->ToString();
+
   check(t->engine, gt_set_init(t->engine, Ap.data(), core.data()), "gt_set_init");
 
   int iter = 0;
