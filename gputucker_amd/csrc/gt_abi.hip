@@ -13,8 +13,10 @@
 //                          ComputingBC accumulates a row (update.cuh:73-99)
 //   row_ptr[n][rows_n+1] u64  global CSR of the stream (rows_n = P_n * bd_n)
 //   where[n][nnz]    u32   per-block where_nnz of mode n, blocks concatenated
-//   A[n]             f32   (P_n * bd_n) x RS factor (RS = R rounded up to 4),
+//                          (built on first request: only the test hooks read it)
+//   A[n]             f32   (P_n * bd_n) x RS factor (RS = row_stride(R)),
 //                          padding rows and columns zero
+//   As[n]            u32   split BF16 copy of A[n], 16 words per row (BF16 paths)
 //   Gt[n]            f32   mode-n unfolded core [o][jL][r]
 //   gram             f64   rows x (R(R+1)/2 + R) packed Gram of the current mode
 #include <algorithm>
@@ -615,8 +617,8 @@ uint64_t mode_update(gt_ctx *c, int n, bool fit_partials, bool sync) {
   sa.lambda = c->o.lambda;
   sa.A = c->A[n].p;
   sa.fit_partials = fit_partials ? c->fit_part.p : nullptr;
+  sa.As = c->As[n].p; // the solve refreshes the split copy (BF16 precisions)
   const uint64_t parts = launch_solve(c->R, sa, c->s);
-  refresh_split(c, n);
   GT_HIP(hipEventRecord(c->ev_mode[n][2], c->s));
   c->mode_pending[n] = true;
   if (sync) {
@@ -726,6 +728,36 @@ double iteration(gt_ctx *c) {
   return 1.0 - std::sqrt(err) / c->norm;
 }
 
+// Per-block where_nnz of mode n (block.tpp:114-186), blocks concatenated:
+// the stream order of mode n (a stable sort of the block-ordered nonzeros by
+// row) partitioned stably by block, as positions inside each block.  Built
+// only for the test hooks and the driver's host blocks.
+void build_where(gt_ctx *c, int n) {
+  const uint64_t nnz = c->nnz;
+  hipStream_t s = c->s;
+  DevBuf<uint32_t> keys, vals, tk, tv;
+  keys.alloc(nnz);
+  vals.alloc(nnz);
+  tk.alloc(nnz);
+  tv.alloc(nnz);
+  DevBuf<char> temp;
+  temp.alloc(radix_sort_temp_bytes(nnz) + 4096);
+  GT_HIP(hipMemcpyAsync(keys.p, c->coo_idx[n].p, nnz * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                        s));
+  iota_u32(vals.p, nnz, s);
+  radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(c->rows[n]), temp.p, s);
+  c->where[n].alloc(nnz);
+  if (c->B == 1) {
+    GT_HIP(hipMemcpyAsync(c->where[n].p, vals.p, nnz * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                          s));
+  } else {
+    keys_from_perm(vals.p, c->coo_blk.p, nnz, keys.p, s);
+    radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(c->B), temp.p, s);
+    local_positions(vals.p, c->coo_blk.p, c->d_block_off.p, nnz, c->where[n].p, s);
+  }
+  GT_HIP(hipStreamSynchronize(s));
+}
+
 void do_partition(gt_ctx *c) {
   require(c->have_coo, GT_ERR_STATE, "gt_set_coo first");
   const int N = c->N;
@@ -820,16 +852,7 @@ void do_partition(gt_ctx *c) {
     // global row pointer of the stream, from the sorted rows (keys)
     c->row_ptr[n].alloc(rows + 1);
     row_ptr_from_sorted(keys.p, nnz, rows, c->row_ptr[n].p, s);
-    // per-block where_nnz: stable partition of the stream order by block
-    c->where[n].alloc(nnz);
-    if (c->B == 1) {
-      GT_HIP(hipMemcpyAsync(c->where[n].p, vals.p, nnz * sizeof(uint32_t),
-                            hipMemcpyDeviceToDevice, s));
-    } else {
-      keys_from_perm(vals.p, c->coo_blk.p, nnz, keys.p, s);
-      radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(c->B), temp.p, s);
-      local_positions(vals.p, c->coo_blk.p, c->d_block_off.p, nnz, c->where[n].p, s);
-    }
+    c->where[n].reset(); // built on demand (build_where): the ALS path never reads it
     GT_HIP(hipStreamSynchronize(s)); // keys, vals and temp are reused
   }
   double ss = 0.0;
@@ -1119,9 +1142,11 @@ int gt_partition_dump(gt_ctx *ctx, uint64_t block, int mode, uint64_t *row_ptr, 
       if (a == mode) coord = rem % ctx->plan.part[a];
       rem /= ctx->plan.part[a];
     }
-    if (where && n)
+    if (where && n) {
+      if (!ctx->where[mode].p) build_where(ctx, mode);
       GT_HIP(hipMemcpyAsync(where, ctx->where[mode].p + off, n * sizeof(uint32_t),
                             hipMemcpyDeviceToHost, ctx->s));
+    }
     if (row_ptr) {
       DevBuf<uint32_t> loc, cnt;
       DevBuf<uint64_t> rp;

@@ -36,6 +36,7 @@
 // the zero eigenvalues of sum dd^T by about lambda.  The accumulators are
 // flushed on row change (complete rows to the Gram table, chunk-straddling
 // runs to the partial tables for the fix-up).
+#include "gt_bf16.hpp"
 #include "gt_contract.hpp"
 
 namespace gt {
@@ -45,8 +46,6 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef short s8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-typedef float fv2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
@@ -60,18 +59,12 @@ constexpr int kDP = 72;
 // DPP row rotation (within each row of 16 lanes) of an FP64 value
 template <int CTRL> __device__ inline double dpp_f64(double v) {
   const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)u, CTRL, 0xF, 0xF, false);
-  const uint32_t hi = __builtin_amdgcn_update_dpp(0u, (uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  // all rows and banks enabled and every source lane valid: no old value needed
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, CTRL, 0xF, 0xF, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, true);
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 constexpr int kDppRowRor4 = 0x124, kDppRowRor8 = 0x128;
-
-// two floats to a BF16 word pair (a in the low half), round to nearest even
-__device__ inline uint32_t pk_bf16(float a, float b) {
-  return __builtin_bit_cast(uint32_t, __builtin_convertvector((fv2{a, b}), bf2));
-}
-__device__ inline float bf16_low_f(uint32_t p) { return __uint_as_float(p << 16); }
-__device__ inline float bf16_high_f(uint32_t p) { return __uint_as_float(p & 0xFFFF0000u); }
 
 template <int R> __device__ inline int packed_index(int p, int q) {
   if (p <= q && q < R) return p * R - p * (p - 1) / 2 + (q - p);
@@ -172,14 +165,13 @@ template <int R> struct ChunkGram {
         if (lo >= hi) continue;
         const int m = mb * 16 + 4 * gk + gbeta;
         const bool in = m >= s && m < e_run;
-        const bool whole = s <= mb * 16 && e_run >= mb * 16 + 16; // the run covers the group
-        // (selects, not a branch: MFMA ignores EXEC, so no MFMA may sit under
+        // (a select, not a branch: MFMA ignores EXEC, so no MFMA may sit under
         // a condition the compiler could turn into an EXEC mask)
         double xa[T], xb[T];
 #pragma unroll
         for (int t = 0; t < T; ++t) {
           xb[t] = (double)X[mb][t];
-          xa[t] = whole ? xb[t] : (double)(in ? X[mb][t] : 0.f);
+          xa[t] = (double)(in ? X[mb][t] : 0.f);
         }
         int p = 0;
 #pragma unroll
@@ -471,26 +463,7 @@ __global__ void __launch_bounds__(256) k_split_factor(const float *__restrict__ 
   const int g = (int)(t & 3);
   if (i >= rows) return;
   const float *row = A + i * RS;
-  auto col = [&](int j) { return j < RS ? row[j] : 0.f; };
-  auto hi2 = [&](int j) { return pk_bf16(col(j), col(j + 1)); };
-  auto lo2 = [&](int j) {
-    const uint32_t h = hi2(j);
-    return pk_bf16(col(j) - bf16_low_f(h), col(j + 1) - bf16_high_f(h));
-  };
-  u4 v = {0u, 0u, 0u, 0u};
-  switch (bf16r_plan(R, g)) {
-  case kPlanHL:
-    if (4 * g < RS) v = u4{hi2(4 * g), hi2(4 * g + 2), lo2(4 * g), lo2(4 * g + 2)};
-    break;
-  case kPlanHH:
-    v = u4{hi2(0), hi2(2), hi2(4), hi2(6)};
-    break;
-  case kPlanR10:
-    v = u4{hi2(8), hi2(10), lo2(8), hi2(8)};
-    break;
-  default:
-    break;
-  }
+  const u4 v = split_words<R>([&](int j) { return j < R ? row[j] : 0.f; }, g);
   *reinterpret_cast<u4 *>(As + t * 4) = v;
 }
 

@@ -190,6 +190,7 @@ struct SolveArgs {
   double lambda;
   float *A; // rows x R, written
   double *fit_partials; // null, or solve_partials(R, rows) residual partials (FP64)
+  uint32_t *As;         // null, or the split BF16 copy of A to refresh (kSplitWords per row)
 };
 // Number of fit partials launch_solve writes for `rows` rows.
 uint64_t solve_partials(int R, uint64_t rows);

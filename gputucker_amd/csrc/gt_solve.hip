@@ -15,7 +15,7 @@
 // the contraction with every other (already updated) factor, so a_i . delta_e
 // is the reconstruction of reconstruction.cuh:58-81 and the fit needs no
 // pass over the nonzeros.
-#include "gt_internal.hpp"
+#include "gt_bf16.hpp"
 
 namespace gt {
 namespace {
@@ -35,6 +35,10 @@ __global__ void __launch_bounds__(64) k_solve_lane(SolveArgs a) {
     if (a.row_ptr[row + 1] == a.row_ptr[row]) {
 #pragma unroll
       for (int r = 0; r < R; ++r) out[r] = 0.f;
+      if (a.As) {
+#pragma unroll
+        for (int w = 0; w < kSplitWords; ++w) a.As[row * kSplitWords + w] = 0u;
+      }
     } else {
       const double *g = a.gram + row * W;
       double L[R * (R + 1) / 2]; // packed lower: L(i, j) at i(i+1)/2 + j
@@ -83,6 +87,14 @@ __global__ void __launch_bounds__(64) k_solve_lane(SolveArgs a) {
       for (int r = 0; r < R; ++r) {
         af[r] = (float)y[r];
         out[r] = af[r];
+      }
+      if constexpr (R <= 15) {
+        if (a.As) { // the row's split BF16 copy for the MFMA delta
+          auto col = [&](int j) { return j < R ? af[j] : 0.f; };
+#pragma unroll
+          for (int g = 0; g < 4; ++g)
+            *reinterpret_cast<gt_u4 *>(a.As + row * kSplitWords + 4 * g) = split_words<R>(col, g);
+        }
       }
       if (a.fit_partials) { // a^T (L L^T) a - lambda |a|^2 - 2 a^T c
 #pragma unroll
