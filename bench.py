@@ -108,11 +108,12 @@ def cpu_baseline(idx, vals, dims, part, R, lam, A, G, target_s=20.0):
                         f"per-nnz {per_nnz * 1e9:.1f} ns, per-mode row cost {rows_cost:.2f} s"))
 
 
-def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup, chunk_tiles):
+def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup, chunk_tiles,
+               budget=0):
     """Partitions, initialises and times `steps` mode updates; returns a dict."""
     N = len(dims)
     e = gt.Engine(N, R, partition=part, lam=lam, precision=precision, chunk_tiles=chunk_tiles,
-                  profile=True)
+                  profile=True, device_budget=budget)
     t0 = time.perf_counter()
     e.set_coo(idx, vals, dims)
     e.partition()
@@ -138,8 +139,10 @@ def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup
     launches = tm["gram_kernel_launches"] - base["gram_kernel_launches"]
     k_ms = (tm["gram_kernel_ms_total"] - base["gram_kernel_ms_total"]) / max(launches, 1)
     t_mode = statistics.median(it_times) / N
+    info = e.info()
     e.close()
     return dict(t_mode=t_mode, value=idx.shape[1] / t_mode, fit=fit, fit_recon=fit_recon,
+                out_of_core=info["out_of_core"], segment_nnz=info["segment_nnz"],
                 kernel_ms=k_ms,
                 launches=launches, total=total, setup_s=setup_s, fit_s=t_fit,
                 partition_ms=tm["ms_partition"])
@@ -155,6 +158,9 @@ def main(argv=None):
     ap.add_argument("--compare", default="fp32", help="baseline GPU path ('' to skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk-tiles", type=int, default=0)
+    ap.add_argument("--budget", type=int, default=0,
+                    help="device budget in bytes; below the resident size the streams are "
+                         "staged from pinned host memory every mode update (PCIe-inclusive)")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -181,10 +187,11 @@ def main(argv=None):
     runs = {}
     for prec in [p for p in (a.compare, a.precision) if p]:
         r = run_engine(gt, idx, vals, dims, A, G, cfg["part"], R, lam, prec, steps, warmup,
-                       a.chunk_tiles)
+                       a.chunk_tiles, a.budget)
         runs[prec] = r
         print(f"[bench] {prec}: t_mode {r['t_mode'] * 1e3:.3f} ms ({r['value']:.3e} nnz/s), "
-              f"gram kernel {r['kernel_ms']:.3f} ms avg over {r['launches']}, fit {r['fit']:.7f} "
+              f"gram kernel {r['kernel_ms']:.3f} ms avg over {r['launches']}, "
+              f"out of core {r['out_of_core']} (segment {r['segment_nnz']}), fit {r['fit']:.7f} "
               f"(reconstruction {r['fit_recon']:.7f}), setup {r['setup_s']:.1f}s "
               f"(partition {r['partition_ms']:.0f} ms), reconstruction pass "
               f"{r['fit_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
@@ -210,7 +217,8 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": cfg["name"], "nnz": int(idx.shape[1]), "order": N, "rank": R,
                    "partition": cfg["part"], "precision": a.precision,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   "out_of_core": bool(head["out_of_core"])},
         "fit": head["fit"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
