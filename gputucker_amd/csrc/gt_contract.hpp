@@ -15,6 +15,14 @@ __device__ inline __attribute__((always_inline)) void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The wavefront's index in its workgroup, as a wave-uniform value: without
+// readfirstlane the compiler treats threadIdx.x / 64 as per-lane, and every
+// chunk bound, tile position and run boundary derived from it would live in
+// vector registers, with 64-bit VALU arithmetic and EXEC-masked branches.
+__device__ inline __attribute__((always_inline)) int wave_index() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+}
+
 typedef float gt_f4 __attribute__((ext_vector_type(4)));
 
 // row i of a factor (stride row_stride(R), zero padded) into registers:

@@ -32,7 +32,7 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
   __shared__ float s_tile[4][kWave * RP];
   __shared__ uint32_t s_row[4][kWave];
   extern __shared__ float s_rows_dyn[];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * 4 + w;
   if (c >= a.chunk_end) return;
   float *lrows = s_rows_dyn + (size_t)w * contract_lds_rows<R>(a.slots) * R * kWave;
@@ -167,7 +167,7 @@ constexpr uint64_t kLongRow = 8; // pieces above which all waves share a row
 template <int R>
 __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_pieces(GramArgs a) {
   constexpr int W = gram_width(R);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
   const uint32_t none = 0xFFFFFFFFu;
   const uint32_t t = c < a.nchunks ? a.chunk_tail[c] : none;               // c -> c + 1
@@ -194,7 +194,7 @@ template <int R>
 __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_rows(GramArgs a) {
   constexpr int W = gram_width(R);
   __shared__ double red[kFixWaves][W];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
   const uint32_t none = 0xFFFFFFFFu;
   const uint32_t t = c < a.nchunks ? a.chunk_tail[c] : none;
@@ -236,7 +236,7 @@ template <int R>
 __global__ void __launch_bounds__(256) k_coo_contract(CooArgs a) {
   __shared__ double red[256];
   extern __shared__ float s_rows_dyn[];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   float *lrows = s_rows_dyn + (size_t)w * contract_lds_rows<R>(a.slots) * R * kWave;
   double err = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;

@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(256) k_gram_valu(GramArgs a, int wpb) {
   __shared__ float s_tile[4][16 * kDP];
   __shared__ uint32_t s_row[4][kWave];
   extern __shared__ __align__(16) float dyn[];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const int slots = a.slots;
   const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
   if (c >= a.chunk_end) return;
@@ -258,7 +258,7 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
   __shared__ float s_tile[4][16 * kDP];
   __shared__ uint32_t s_row[4][kWave];
   extern __shared__ __align__(16) float dyn[];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const int slots = a.slots;
   // the core operand tables go to LDS once per workgroup:
   // kBf16: K16 hi table; kBf16R: K32 table, then (R > 10) the K16 lo table
