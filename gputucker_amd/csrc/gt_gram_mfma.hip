@@ -89,6 +89,7 @@ template <int R> struct ChunkGram {
   static constexpr int W = gram_width(R);
   static constexpr int T = (R + 4) / 4; // column tiles covering columns 0..R
   static constexpr int NP = T * (T + 1) / 2;
+  static constexpr int NS = (NP + 3) / 4; // block pairs stored per lane at flush
   const GramArgs &a;
   uint64_t c, cb, ce;
   int lane, gk, gbeta, gx;
@@ -96,11 +97,27 @@ template <int R> struct ChunkGram {
   uint32_t cur, last_row;
   uint64_t run_begin;
   double acc[NP];
+  // Gram-row offset this lane stores for pair 4k + gbeta, or -1: lane
+  // 16i + 4beta + j holds entry (4ta + i, 4tc + j) of pair (ta, tc)
+  int st_at[NS];
 
   __device__ ChunkGram(const GramArgs &args, uint64_t chunk, uint64_t b, uint64_t e, int l)
       : a(args), c(chunk), cb(b), ce(e), lane(l), gk(l >> 4), gbeta((l >> 2) & 3), gx(l & 3) {
 #pragma unroll
     for (int p = 0; p < NP; ++p) acc[p] = 0.0;
+    const int fi = lane >> 4, fj = lane & 3;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      int at = -1, p = 0;
+#pragma unroll
+      for (int ta = 0; ta < T; ++ta)
+#pragma unroll
+        for (int tc = ta; tc < T; ++tc, ++p) {
+          const int P = 4 * ta + fi, Q = 4 * tc + fj;
+          if (p == 4 * k + gbeta && P <= Q) at = packed_index<R>(P, Q);
+        }
+      st_at[k] = at;
+    }
     cur = __builtin_amdgcn_readfirstlane(a.idx_row[cb]);
     run_begin = cb;
     // Does the chunk's first row start before it, or its last row end after
@@ -120,22 +137,23 @@ template <int R> struct ChunkGram {
       dst = a.bpart0 + (size_t)c * W;
     else
       dst = a.bpart1 + (size_t)c * W;
-    // add the four block partials (lanes 4 apart inside each row of 16);
-    // quad beta then stores the block pairs p with p % 4 == beta
-    const int fi = lane >> 4, fj = lane & 3;
-    int p = 0;
+    // add the four block partials (lanes 4 apart inside each row of 16),
+    // every pair's step before the next step so the pairs' DPP chains
+    // interleave; quad beta then stores the block pairs p with p % 4 == beta
 #pragma unroll
-    for (int ta = 0; ta < T; ++ta)
+    for (int p = 0; p < NP; ++p) acc[p] += dpp_f64<kDppRowRor8>(acc[p]);
 #pragma unroll
-      for (int tc = ta; tc < T; ++tc, ++p) {
-        double v = acc[p];
-        v += dpp_f64<kDppRowRor8>(v);
-        v += dpp_f64<kDppRowRor4>(v);
-        const int P = 4 * ta + fi, Q = 4 * tc + fj;
-        const int at = (P <= Q) ? packed_index<R>(P, Q) : -1;
-        if ((p & 3) == gbeta && at >= 0) dst[at] = v;
-        acc[p] = 0.0;
-      }
+    for (int p = 0; p < NP; ++p) acc[p] += dpp_f64<kDppRowRor4>(acc[p]);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      double v = acc[4 * k];
+#pragma unroll
+      for (int b = 1; b < 4; ++b)
+        if (4 * k + b < NP) v = (gbeta == b) ? acc[4 * k + b] : v;
+      if (st_at[k] >= 0) dst[st_at[k]] = v;
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) acc[p] = 0.0;
   }
 
   // Adds tile [tb, tb + cnt) (values in `tile`, row ids in `srow`); the
