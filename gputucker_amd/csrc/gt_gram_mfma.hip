@@ -167,6 +167,23 @@ template <int R> struct ChunkGram {
     const uint32_t myrow = srow[lane];
     const uint32_t prev = lane ? srow[lane - 1] : cur;
     const uint64_t starts = __ballot(lane < cnt && myrow != prev);
+    if (starts == 0 && cnt == kWave) {
+      // the whole tile continues the current run (the common case inside
+      // long rows): no masks.  The condition is wave-uniform, a scalar branch.
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        double xb[T];
+#pragma unroll
+        for (int t = 0; t < T; ++t) xb[t] = (double)X[mb][t];
+        int p = 0;
+#pragma unroll
+        for (int ta = 0; ta < T; ++ta)
+#pragma unroll
+          for (int tc = ta; tc < T; ++tc, ++p)
+            acc[p] = __builtin_amdgcn_mfma_f64_4x4x4f64(xb[ta], xb[tc], acc[p], 0, 0, 0);
+      }
+      return;
+    }
     int s = 0;
     while (s < cnt) {
       if ((starts >> s) & 1ull) {
