@@ -143,7 +143,8 @@ def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup
     e.close()
     return dict(t_mode=t_mode, value=idx.shape[1] / t_mode, fit=fit, fit_recon=fit_recon,
                 out_of_core=info["out_of_core"], segment_nnz=info["segment_nnz"],
-                kernel_ms=k_ms,
+                kernel_ms=k_ms, gram_phase_ms=statistics.mean(tm["ms_gram"]),
+                solve_ms=statistics.mean(tm["ms_solve"]),
                 launches=launches, total=total, setup_s=setup_s, fit_s=t_fit,
                 partition_ms=tm["ms_partition"])
 
@@ -190,7 +191,8 @@ def main(argv=None):
                        a.chunk_tiles, a.budget)
         runs[prec] = r
         print(f"[bench] {prec}: t_mode {r['t_mode'] * 1e3:.3f} ms ({r['value']:.3e} nnz/s), "
-              f"gram kernel {r['kernel_ms']:.3f} ms avg over {r['launches']}, "
+              f"gram kernel {r['kernel_ms']:.3f} ms avg over {r['launches']} "
+              f"(gram phase {r['gram_phase_ms']:.3f} ms, solve {r['solve_ms']:.3f} ms), "
               f"out of core {r['out_of_core']} (segment {r['segment_nnz']}), fit {r['fit']:.7f} "
               f"(reconstruction {r['fit_recon']:.7f}), setup {r['setup_s']:.1f}s "
               f"(partition {r['partition_ms']:.0f} ms), reconstruction pass "

@@ -20,15 +20,16 @@ namespace gt {
 namespace {
 
 template <int R> struct GramLayout {
-  static constexpr int W = gram_width(R);
-  static constexpr int E = (W + kWave - 1) / kWave;      // entries per lane
+  static constexpr int W = gram_width(R);                // row stride
+  static constexpr int NE = gram_entries(R);
+  static constexpr int E = (NE + kWave - 1) / kWave;     // entries per lane
   static constexpr int RP = (R % 2 == 0) ? R + 1 : R + 2; // odd tile pitch; column R holds x
 };
 
 template <int R>
 __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
   using Lay = GramLayout<R>;
-  constexpr int W = Lay::W, E = Lay::E, RP = Lay::RP;
+  constexpr int W = Lay::W, NE = Lay::NE, E = Lay::E, RP = Lay::RP;
   __shared__ float s_tile[4][kWave * RP];
   __shared__ uint32_t s_row[4][kWave];
   extern __shared__ float s_rows_dyn[];
@@ -47,7 +48,7 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
 #pragma unroll
   for (int qi = 0; qi < E; ++qi) {
     const int q = lane + qi * kWave;
-    act[qi] = q < W;
+    act[qi] = q < NE;
     int p = 0, p2 = 0;
     if (q < R * (R + 1) / 2) {
       int rem = q;
@@ -56,7 +57,7 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
         ++p;
       }
       p2 = p + rem;
-    } else if (q < W) {
+    } else if (q < NE) {
       p = q - R * (R + 1) / 2;
       p2 = R;
     }
@@ -166,7 +167,7 @@ constexpr uint64_t kLongRow = 8; // pieces above which all waves share a row
 
 template <int R>
 __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_pieces(GramArgs a) {
-  constexpr int W = gram_width(R);
+  constexpr int W = gram_width(R), NE = gram_entries(R);
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
   const uint32_t none = 0xFFFFFFFFu;
@@ -185,15 +186,15 @@ __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_pieces(GramArgs 
     if (e == j) continue;
     const uint64_t cj = (uint64_t)blockIdx.x * kPiece + j;
     double *first = (a.row_ptr[r] <= cj * a.chunk ? a.bpart0 : a.bpart1) + cj * W;
-    for (int q = lane; q < W; q += kWave)
+    for (int q = lane; q < NE; q += kWave)
       first[q] += sum_bpart0(a.bpart0, cj + 1, cj + (e - j), 1, W, q);
   }
 }
 
 template <int R>
 __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_rows(GramArgs a) {
-  constexpr int W = gram_width(R);
-  __shared__ double red[kFixWaves][W];
+  constexpr int W = gram_width(R), NE = gram_entries(R);
+  __shared__ double red[kFixWaves][NE];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
   const uint32_t none = 0xFFFFFFFFu;
@@ -211,19 +212,19 @@ __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_rows(GramArgs a)
     const uint64_t p0 = cj / kPiece, pl = last / kPiece;
     const double *first = (rb == cj * a.chunk ? a.bpart0 : a.bpart1) + cj * W;
     if (pl - p0 > kLongRow) { // the same decision in every wave
-      for (int q = lane; q < W; q += kWave)
+      for (int q = lane; q < NE; q += kWave)
         red[w][q] =
             sum_bpart0(a.bpart0, (p0 + 1 + w) * kPiece, pl * kPiece, kPiece * kFixWaves, W, q);
       __syncthreads();
       if (w == 0) {
-        for (int q = lane; q < W; q += kWave)
+        for (int q = lane; q < NE; q += kWave)
           a.gram[(size_t)r * W + q] =
               first[q] + ((red[0][q] + red[1][q]) + (red[2][q] + red[3][q]));
       }
       __syncthreads();
     } else {
       if ((k++ % kFixWaves) != w) continue;
-      for (int q = lane; q < W; q += kWave) {
+      for (int q = lane; q < NE; q += kWave) {
         double s = first[q];
         if (pl > p0) s += sum_bpart0(a.bpart0, (p0 + 1) * kPiece, pl * kPiece, kPiece, W, q);
         a.gram[(size_t)r * W + q] = s;

@@ -19,8 +19,12 @@ constexpr int kWave = 64;      // CDNA wavefront width
 constexpr int kTile = 64;      // nonzeros per tile: one per lane
 
 // Packed Gram row: upper triangle of B (R(R+1)/2 doubles, row-major p <= q)
-// followed by C (R doubles).
-__host__ __device__ constexpr int gram_width(int R) { return R * (R + 1) / 2 + R; }
+// followed by C (R doubles): gram_entries(R) values.  Rows are stored with
+// the stride gram_width(R), rounded up to an even count so every row starts
+// on a 16-byte boundary and is read with 16-byte loads; the padding entry is
+// never read.
+__host__ __device__ constexpr int gram_entries(int R) { return R * (R + 1) / 2 + R; }
+__host__ __device__ constexpr int gram_width(int R) { return (gram_entries(R) + 1) / 2 * 2; }
 
 // Factor rows are stored with a stride of 4 or 8 floats (R <= 8), else R
 // rounded up to 16 floats, padding zero: a row is read with 16-byte loads and,

@@ -28,11 +28,60 @@ __host__ __device__ constexpr int up_index(int R, int p, int q) {
 template <int R>
 __global__ void __launch_bounds__(64) k_solve_lane(SolveArgs a) {
   constexpr int W = gram_width(R);
-  const uint64_t row = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int kPiece = 16; // rows staged through LDS at a time
+  __shared__ __align__(16) double buf[kPiece * W];
+  const int lane = threadIdx.x;
+  const uint64_t row0 = (uint64_t)blockIdx.x * kWave;
+  const uint64_t row = row0 + lane;
+  // The wave's 64 Gram rows are contiguous.  They are read 16 rows at a time
+  // as one block of 16-byte loads (whole cache lines, every byte used once)
+  // into LDS, and the 16 lanes that own those rows copy theirs out; a
+  // lane-per-row read would touch 64 lines per load instruction and refetch
+  // them as they are evicted.
+  // the row's degree is read first, so its latency overlaps the staging
+  const bool empty = row < a.rows && a.row_ptr[row + 1] == a.row_ptr[row];
+  double L[R * (R + 1) / 2]; // packed lower: L(i, j) at i(i+1)/2 + j
+  double cv[R];
+#pragma unroll
+  for (int k = 0; k < kWave / kPiece; ++k) {
+    const uint64_t pr = row0 + (uint64_t)k * kPiece;
+    const int n = pr < a.rows ? (int)(a.rows - pr < (uint64_t)kPiece ? a.rows - pr : kPiece) : 0;
+    if (n > 0) { // the piece in two batches of loads in flight, each then written to LDS
+      constexpr int kCopy = (kPiece * W / 2 + kWave - 1) / kWave, kHalf = (kCopy + 1) / 2;
+      const double2 *src = reinterpret_cast<const double2 *>(a.gram + pr * W);
+      double2 *dst = reinterpret_cast<double2 *>(buf);
+      const int n2 = n * W / 2;
+#pragma unroll
+      for (int h = 0; h < kCopy; h += kHalf) {
+        double2 t[kHalf];
+#pragma unroll
+        for (int c = 0; c < kHalf; ++c) {
+          const int i = lane + (h + c) * kWave;
+          t[c] = (h + c < kCopy && i < n2) ? src[i] : double2{0.0, 0.0};
+        }
+#pragma unroll
+        for (int c = 0; c < kHalf; ++c) {
+          const int i = lane + (h + c) * kWave;
+          if (h + c < kCopy && i < n2) dst[i] = t[c];
+        }
+      }
+    }
+    __syncthreads();
+    if (lane / kPiece == k && lane % kPiece < n) {
+      const double *g = buf + (lane % kPiece) * W;
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) L[i * (i + 1) / 2 + j] = g[up_index(R, j, i)];
+#pragma unroll
+      for (int i = 0; i < R; ++i) cv[i] = g[R * (R + 1) / 2 + i];
+    }
+    __syncthreads();
+  }
   double contrib = 0.0;
   if (row < a.rows) {
     float *out = a.A + row * row_stride(R);
-    if (a.row_ptr[row + 1] == a.row_ptr[row]) {
+    if (empty) {
 #pragma unroll
       for (int r = 0; r < R; ++r) out[r] = 0.f;
       if (a.As) {
@@ -40,15 +89,6 @@ __global__ void __launch_bounds__(64) k_solve_lane(SolveArgs a) {
         for (int w = 0; w < kSplitWords; ++w) a.As[row * kSplitWords + w] = 0u;
       }
     } else {
-      const double *g = a.gram + row * W;
-      double L[R * (R + 1) / 2]; // packed lower: L(i, j) at i(i+1)/2 + j
-#pragma unroll
-      for (int i = 0; i < R; ++i)
-#pragma unroll
-        for (int j = 0; j <= i; ++j) L[i * (i + 1) / 2 + j] = g[up_index(R, j, i)];
-      double cv[R];
-#pragma unroll
-      for (int i = 0; i < R; ++i) cv[i] = g[R * (R + 1) / 2 + i];
 #pragma unroll
       for (int i = 0; i < R; ++i) L[i * (i + 1) / 2 + i] += a.lambda;
 #pragma unroll
