@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -203,6 +204,13 @@ struct gt_ctx {
   uint64_t seg_nnz = 0;
   uint32_t *h_idx[GT_MAX_ORDER][GT_MAX_ORDER] = {};
   float *h_val[GT_MAX_ORDER] = {};
+  // packed host streams (every factor has at most 2^24 rows): mode n's
+  // stream keeps its other modes' indices in 24 bits, three words per four
+  // nonzeros, and drops its row index, which row_ptr[n] rebuilds on the device
+  bool packed = false;
+  uint32_t stream_bytes = 0; // host bytes per nonzero per mode
+  uint32_t *h_pack[GT_MAX_ORDER][GT_MAX_ORDER] = {};
+  DevBuf<uint32_t> stage_pack[2][GT_MAX_ORDER];
   DevBuf<uint32_t> stage_idx[2][GT_MAX_ORDER];
   DevBuf<float> stage_val[2];
   hipStream_t s_copy = nullptr;
@@ -229,17 +237,23 @@ struct gt_ctx {
     if (s) (void)hipStreamSynchronize(s); // queued copies may still read the buffers
     if (s_copy) (void)hipStreamSynchronize(s_copy);
     for (int n = 0; n < GT_MAX_ORDER; ++n) {
-      for (int k = 0; k < GT_MAX_ORDER; ++k)
+      for (int k = 0; k < GT_MAX_ORDER; ++k) {
         if (h_idx[n][k]) {
           (void)hipHostFree(h_idx[n][k]);
           h_idx[n][k] = nullptr;
         }
+        if (h_pack[n][k]) {
+          (void)hipHostFree(h_pack[n][k]);
+          h_pack[n][k] = nullptr;
+        }
+      }
       if (h_val[n]) {
         (void)hipHostFree(h_val[n]);
         h_val[n] = nullptr;
       }
     }
     streaming = false;
+    packed = false;
   }
   // Reads the event times of mode updates whose events have completed (the
   // caller has synchronised the stream).
@@ -449,16 +463,37 @@ void maybe_stream(gt_ctx *c) {
   fixed += (uint64_t)N * ipow(R, N) * 4 * 2 + (1u << 20);
   if (movable + fixed <= budget) return; // fits: stay resident
   if (fixed >= budget) throw GtError(GT_ERR_MEMORY, "device budget is below the per-row state");
-  uint64_t seg = (budget - fixed) / (2ull * (N + 1) * 4);
+  // packing needs every index below 2^24; GT_STREAM_PACK=0 turns it off
+  const char *pack_env = std::getenv("GT_STREAM_PACK");
+  bool packed = !(pack_env && pack_env[0] == '0');
+  for (int k = 0; k < N; ++k) packed = packed && c->rows[k] <= (1ull << 24);
+  // per nonzero and staging buffer: the unpacked indices and the value, plus
+  // the packed words when packing
+  const uint64_t stage_bytes = 4ull * (N + 1) + (packed ? 3ull * N : 0);
+  uint64_t seg = (budget - fixed) / (2ull * stage_bytes);
   if (c->plan.avail) seg = std::min(seg, c->plan.avail);
   seg = (seg / c->chunk) * c->chunk;
   if (seg == 0) throw GtError(GT_ERR_MEMORY, "device budget too small for one staged chunk");
   (void)W;
+  const uint64_t groups = (nnz + 3) / 4;
+  DevBuf<uint32_t> tmp;
+  if (packed) tmp.alloc(3 * groups);
   for (int n = 0; n < N; ++n) {
     for (int k = 0; k < N; ++k) {
-      GT_HIP(hipHostMalloc(&c->h_idx[n][k], nnz * sizeof(uint32_t), hipHostMallocDefault));
-      GT_HIP(hipMemcpy(c->h_idx[n][k], c->st_idx[n][k].p, nnz * sizeof(uint32_t),
-                       hipMemcpyDeviceToHost));
+      if (packed) {
+        if (k != n) {
+          pack24(c->st_idx[n][k].p, nnz, tmp.p, c->s);
+          GT_HIP(hipHostMalloc(&c->h_pack[n][k], 3 * groups * sizeof(uint32_t),
+                               hipHostMallocDefault));
+          GT_HIP(hipMemcpyAsync(c->h_pack[n][k], tmp.p, 3 * groups * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost, c->s));
+          GT_HIP(hipStreamSynchronize(c->s));
+        }
+      } else {
+        GT_HIP(hipHostMalloc(&c->h_idx[n][k], nnz * sizeof(uint32_t), hipHostMallocDefault));
+        GT_HIP(hipMemcpy(c->h_idx[n][k], c->st_idx[n][k].p, nnz * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost));
+      }
       c->st_idx[n][k].reset();
     }
     GT_HIP(hipHostMalloc(&c->h_val[n], nnz * sizeof(float), hipHostMallocDefault));
@@ -469,8 +504,12 @@ void maybe_stream(gt_ctx *c) {
   for (int k = 0; k < N; ++k) c->coo_idx[k].reset();
   c->coo_val.reset();
   c->coo_blk.reset();
+  tmp.reset();
   for (int b = 0; b < 2; ++b) {
-    for (int k = 0; k < N; ++k) c->stage_idx[b][k].alloc(seg);
+    for (int k = 0; k < N; ++k) {
+      c->stage_idx[b][k].alloc(seg);
+      if (packed) c->stage_pack[b][k].alloc(3 * (seg / 4));
+    }
     c->stage_val[b].alloc(seg);
     if (!c->ev_copy[b]) GT_HIP(hipEventCreateWithFlags(&c->ev_copy[b], hipEventDisableTiming));
     if (!c->ev_comp[b]) GT_HIP(hipEventCreateWithFlags(&c->ev_comp[b], hipEventDisableTiming));
@@ -479,6 +518,8 @@ void maybe_stream(gt_ctx *c) {
   const uint64_t nseg = (nnz + seg - 1) / seg;
   c->err_part.alloc(std::max<uint64_t>(4096, nseg * kErrPartialsPerSegment));
   c->seg_nnz = seg;
+  c->packed = packed;
+  c->stream_bytes = packed ? 3u * (N - 1) + 4u : 4u * (N + 1);
   c->streaming = true;
 }
 
@@ -560,9 +601,21 @@ std::pair<uint64_t, uint64_t> stage_segment(gt_ctx *c, int n, uint64_t k) {
   // the last kernel that read buffer b (this pass or, since mode updates are
   // queued without a host wait, the previous one) must be done with it
   GT_HIP(hipStreamWaitEvent(c->s_copy, c->ev_comp[b], 0));
-  for (int m = 0; m < c->N; ++m)
-    GT_HIP(hipMemcpyAsync(c->stage_idx[b][m].p, c->h_idx[n][m] + lo, len * sizeof(uint32_t),
-                          hipMemcpyHostToDevice, c->s_copy));
+  if (c->packed) { // lo is a multiple of the chunk size, so of four
+    const uint64_t words = 3 * ((len + 3) / 4);
+    for (int m = 0; m < c->N; ++m) {
+      if (m == n) continue;
+      GT_HIP(hipMemcpyAsync(c->stage_pack[b][m].p, c->h_pack[n][m] + lo / 4 * 3,
+                            words * sizeof(uint32_t), hipMemcpyHostToDevice, c->s_copy));
+      unpack24(c->stage_pack[b][m].p, len, c->stage_idx[b][m].p, c->s_copy);
+    }
+    rows_from_ptr(c->row_ptr[n].p, (uint32_t)c->rows[n], lo, len, c->stage_idx[b][n].p,
+                  c->s_copy);
+  } else {
+    for (int m = 0; m < c->N; ++m)
+      GT_HIP(hipMemcpyAsync(c->stage_idx[b][m].p, c->h_idx[n][m] + lo, len * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, c->s_copy));
+  }
   GT_HIP(hipMemcpyAsync(c->stage_val[b].p, c->h_val[n] + lo, len * sizeof(float),
                         hipMemcpyHostToDevice, c->s_copy));
   GT_HIP(hipEventRecord(c->ev_copy[b], c->s_copy));
@@ -1091,6 +1144,7 @@ int gt_get_partition_info(gt_ctx *ctx, gt_partition_info *info) {
     info->avail_nnz_per_task = ctx->plan.avail;
     info->out_of_core = ctx->streaming ? 1 : 0;
     info->segment_nnz = ctx->seg_nnz;
+    info->stream_bytes_per_nnz = ctx->streaming ? ctx->stream_bytes : 0;
   });
 }
 

@@ -118,6 +118,63 @@ __global__ void k_sub_u32(const uint32_t *__restrict__ idx, uint32_t base, uint6
     out[e] = idx[e] - base;
 }
 
+// 24-bit packing of an index array for the out-of-core host streams: every
+// group of four indices (each < 2^24) becomes three words, little-endian
+// bytes in order; a short last group is padded with zeros.
+__global__ void k_pack24(const uint32_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
+  const uint64_t groups = (n + 3) / 4;
+  for (uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < groups;
+       gi += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) v[j] = 4 * gi + j < n ? in[4 * gi + j] : 0u;
+    out[3 * gi + 0] = v[0] | (v[1] << 24);
+    out[3 * gi + 1] = (v[1] >> 8) | (v[2] << 16);
+    out[3 * gi + 2] = (v[2] >> 16) | (v[3] << 8);
+  }
+}
+
+__global__ void k_unpack24(const uint32_t *__restrict__ in, uint64_t groups,
+                           uint32_t *__restrict__ out) {
+  for (uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < groups;
+       gi += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t w0 = in[3 * gi], w1 = in[3 * gi + 1], w2 = in[3 * gi + 2];
+    uint4 v;
+    v.x = w0 & 0xFFFFFFu;
+    v.y = (w0 >> 24) | ((w1 & 0xFFFFu) << 8);
+    v.z = (w1 >> 16) | ((w2 & 0xFFu) << 16);
+    v.w = w2 >> 8;
+    reinterpret_cast<uint4 *>(out)[gi] = v;
+  }
+}
+
+// Row of each stream position e in [lo, lo + len) from the CSR row pointer
+// (rows + 1 entries): the last r with row_ptr[r] <= e, which skips empty
+// rows.  A wave finds the rows of its first and last positions, then each
+// lane searches between them.
+__device__ inline uint32_t row_of(const uint64_t *__restrict__ rp, uint32_t a, uint32_t b,
+                                  uint64_t e) {
+  while (a < b) { // invariant: rp[a] <= e, and rp[b + 1] > e
+    const uint32_t m = a + (b - a + 1) / 2;
+    if (rp[m] <= e)
+      a = m;
+    else
+      b = m - 1;
+  }
+  return a;
+}
+
+__global__ void k_rows_from_ptr(const uint64_t *__restrict__ rp, uint32_t rows, uint64_t lo,
+                                uint64_t len, uint32_t *__restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t w0 = t - (threadIdx.x & 63); // first position of this wave
+  if (w0 >= len) return;
+  const uint64_t w1 = (w0 + 63 < len ? w0 + 63 : len - 1);
+  const uint32_t r0 = row_of(rp, 0, rows - 1, lo + w0);
+  const uint32_t r1 = row_of(rp, r0, rows - 1, lo + w1);
+  if (t < len) out[t] = row_of(rp, r0, r1, lo + t);
+}
+
 } // namespace
 
 void block_ids(const PartArgs &a, uint32_t blocks, uint32_t *keys, uint32_t *vals,
@@ -175,6 +232,30 @@ void local_positions(const uint32_t *pos, const uint32_t *blk, const uint64_t *b
 void sub_u32(const uint32_t *idx, uint32_t base, uint64_t n, uint32_t *out, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_sub_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, idx, base, n, out);
+  GT_HIP(hipGetLastError());
+}
+
+void pack24(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_pack24, dim3(grid_for((n + 3) / 4, 256)), dim3(256), 0, s, in, n, out);
+  GT_HIP(hipGetLastError());
+}
+
+void unpack24(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s) {
+  if (!n) return;
+  const uint64_t groups = (n + 3) / 4;
+  hipLaunchKernelGGL(k_unpack24, dim3(grid_for(groups, 256)), dim3(256), 0, s, in, groups, out);
+  GT_HIP(hipGetLastError());
+}
+
+void rows_from_ptr(const uint64_t *row_ptr, uint32_t rows, uint64_t lo, uint64_t len,
+                   uint32_t *out, hipStream_t s) {
+  if (!len) return;
+  if (!rows) throw std::invalid_argument("rows_from_ptr: no rows");
+  const uint64_t g = (len + 255) / 256;
+  if (g > 0xFFFFFFFFull) throw std::invalid_argument("rows_from_ptr: segment too long");
+  hipLaunchKernelGGL(k_rows_from_ptr, dim3((unsigned)g), dim3(256), 0, s, row_ptr, rows, lo, len,
+                     out);
   GT_HIP(hipGetLastError());
 }
 

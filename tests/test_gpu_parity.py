@@ -218,13 +218,17 @@ def test_skewed_sample_of_benchmark_workload(prec, tol):
     assert abs(fits[0] - f64[0]) / abs(f64[0]) < tol, (fits, f64)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16r"])
-def test_out_of_core_matches_in_core(prec):
+@pytest.mark.parametrize("prec,pack", [("fp32", True), ("bf16r", True), ("fp32", False)])
+def test_out_of_core_matches_in_core(prec, pack, monkeypatch):
     """A device budget below the resident layout stages the per-mode streams
-    from pinned host memory segment by segment (double-buffered copies).  The
-    kernels see the same chunks in the same order, so factors must be
-    bit-identical to the in-core run; the fit sums its partials in another
-    grouping, so it agrees to FP64 rounding."""
+    from pinned host memory segment by segment (double-buffered copies).  With
+    every factor below 2^24 rows the host streams hold 24-bit packed indices
+    and no row index (10 bytes per nonzero for order 3), unpacked and rebuilt
+    from row_ptr on the device; GT_STREAM_PACK=0 ships plain 32-bit indices
+    (16 bytes).  The kernels see the same chunks in the same order, so factors
+    must be bit-identical to the in-core run; the fit sums its partials in
+    another grouping, so it agrees to FP64 rounding."""
+    monkeypatch.setenv("GT_STREAM_PACK", "1" if pack else "0")
     idx, vals, dims = random_coo(11, [300, 200, 100], 30000, skew=1.3)
     A, G = random_init(11, dims, 6)
     kw = dict(precision=prec, chunk_tiles=1, max_iters=2, tol=0.0)
@@ -237,6 +241,7 @@ def test_out_of_core_matches_in_core(prec):
     oc.set_init(A, G)
     info = oc.info()
     assert info["out_of_core"] == 1
+    assert info["stream_bytes_per_nnz"] == (10 if pack else 16)
     assert 0 < info["segment_nnz"] < 30000 and info["segment_nnz"] % 64 == 0
     f_ref, f_oc = ref.run(), oc.run()
     for m in range(3):
@@ -310,6 +315,7 @@ def test_order5_out_of_core_sample(prec):
     oc = make(idx, vals, dims, 5, [1, 1, 1, 1, 1], device_budget=2_000_000, **kw)
     oc.set_init(A, G)
     assert oc.info()["out_of_core"] == 1 and ref.info()["out_of_core"] == 0
+    assert oc.info()["stream_bytes_per_nnz"] == 3 * 4 + 4  # four packed indices and the value
     f_ref, f_oc = ref.run(), oc.run()
     for m in range(5):
         assert np.array_equal(ref.get_factor(m), oc.get_factor(m)), m
