@@ -205,10 +205,12 @@ struct gt_ctx {
   uint32_t *h_idx[GT_MAX_ORDER][GT_MAX_ORDER] = {};
   float *h_val[GT_MAX_ORDER] = {};
   // packed host streams (every factor has at most 2^24 rows): mode n's
-  // stream keeps its other modes' indices in 24 bits, three words per four
-  // nonzeros, and drops its row index, which row_ptr[n] rebuilds on the device
+  // stream keeps each other mode k's indices in pack_bits[k] bits (16 when
+  // factor k has at most 2^16 rows, else 24) and drops its row index, which
+  // row_ptr[n] rebuilds on the device
   bool packed = false;
-  uint32_t stream_bytes = 0; // host bytes per nonzero per mode
+  int pack_bits[GT_MAX_ORDER] = {};
+  uint32_t stream_bytes = 0; // host bytes per nonzero, the largest over the modes
   uint32_t *h_pack[GT_MAX_ORDER][GT_MAX_ORDER] = {};
   DevBuf<uint32_t> stage_pack[2][GT_MAX_ORDER];
   DevBuf<uint32_t> stage_idx[2][GT_MAX_ORDER];
@@ -451,6 +453,10 @@ void ensure_work(gt_ctx *c) {
 // delta.cuh:165-213, with the tasks of scheduler.tpp:85-159 replaced by
 // fixed-size segments of the row-ordered stream).  The partition itself runs
 // with the tensor resident.
+// Words holding n packed indices of `bits` bits (n a multiple of four when
+// it is an offset).
+uint64_t pack_words(int bits, uint64_t n) { return bits == 16 ? (n + 1) / 2 : 3 * ((n + 3) / 4); }
+
 void maybe_stream(gt_ctx *c) {
   const uint64_t budget = c->o.device_budget;
   if (budget == 0) return;
@@ -475,17 +481,20 @@ void maybe_stream(gt_ctx *c) {
   seg = (seg / c->chunk) * c->chunk;
   if (seg == 0) throw GtError(GT_ERR_MEMORY, "device budget too small for one staged chunk");
   (void)W;
-  const uint64_t groups = (nnz + 3) / 4;
+  for (int k = 0; k < N; ++k) c->pack_bits[k] = !packed ? 0 : c->rows[k] <= (1u << 16) ? 16 : 24;
   DevBuf<uint32_t> tmp;
-  if (packed) tmp.alloc(3 * groups);
+  if (packed) tmp.alloc(3 * ((nnz + 3) / 4));
   for (int n = 0; n < N; ++n) {
     for (int k = 0; k < N; ++k) {
       if (packed) {
         if (k != n) {
-          pack24(c->st_idx[n][k].p, nnz, tmp.p, c->s);
-          GT_HIP(hipHostMalloc(&c->h_pack[n][k], 3 * groups * sizeof(uint32_t),
-                               hipHostMallocDefault));
-          GT_HIP(hipMemcpyAsync(c->h_pack[n][k], tmp.p, 3 * groups * sizeof(uint32_t),
+          const uint64_t words = pack_words(c->pack_bits[k], nnz);
+          if (c->pack_bits[k] == 16)
+            pack16(c->st_idx[n][k].p, nnz, tmp.p, c->s);
+          else
+            pack24(c->st_idx[n][k].p, nnz, tmp.p, c->s);
+          GT_HIP(hipHostMalloc(&c->h_pack[n][k], words * sizeof(uint32_t), hipHostMallocDefault));
+          GT_HIP(hipMemcpyAsync(c->h_pack[n][k], tmp.p, words * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, c->s));
           GT_HIP(hipStreamSynchronize(c->s));
         }
@@ -519,7 +528,17 @@ void maybe_stream(gt_ctx *c) {
   c->err_part.alloc(std::max<uint64_t>(4096, nseg * kErrPartialsPerSegment));
   c->seg_nnz = seg;
   c->packed = packed;
-  c->stream_bytes = packed ? 3u * (N - 1) + 4u : 4u * (N + 1);
+  c->stream_bytes = 4u * (N + 1);
+  if (packed) {
+    uint32_t most = 0;
+    for (int n = 0; n < N; ++n) {
+      uint32_t b = 4;
+      for (int k = 0; k < N; ++k)
+        if (k != n) b += (uint32_t)c->pack_bits[k] / 8;
+      most = std::max(most, b);
+    }
+    c->stream_bytes = most;
+  }
   c->streaming = true;
 }
 
@@ -602,12 +621,16 @@ std::pair<uint64_t, uint64_t> stage_segment(gt_ctx *c, int n, uint64_t k) {
   // queued without a host wait, the previous one) must be done with it
   GT_HIP(hipStreamWaitEvent(c->s_copy, c->ev_comp[b], 0));
   if (c->packed) { // lo is a multiple of the chunk size, so of four
-    const uint64_t words = 3 * ((len + 3) / 4);
     for (int m = 0; m < c->N; ++m) {
       if (m == n) continue;
-      GT_HIP(hipMemcpyAsync(c->stage_pack[b][m].p, c->h_pack[n][m] + lo / 4 * 3,
-                            words * sizeof(uint32_t), hipMemcpyHostToDevice, c->s_copy));
-      unpack24(c->stage_pack[b][m].p, len, c->stage_idx[b][m].p, c->s_copy);
+      const int bits = c->pack_bits[m];
+      GT_HIP(hipMemcpyAsync(c->stage_pack[b][m].p, c->h_pack[n][m] + pack_words(bits, lo),
+                            pack_words(bits, len) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                            c->s_copy));
+      if (bits == 16)
+        unpack16(c->stage_pack[b][m].p, len, c->stage_idx[b][m].p, c->s_copy);
+      else
+        unpack24(c->stage_pack[b][m].p, len, c->stage_idx[b][m].p, c->s_copy);
     }
     rows_from_ptr(c->row_ptr[n].p, (uint32_t)c->rows[n], lo, len, c->stage_idx[b][n].p,
                   c->s_copy);

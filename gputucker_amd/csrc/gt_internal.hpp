@@ -113,11 +113,14 @@ void local_positions(const uint32_t *pos, const uint32_t *blk, const uint64_t *b
                      uint64_t n, uint32_t *where, hipStream_t s);
 // out[e] = idx[e] - base  (block-local index of one block)
 void sub_u32(const uint32_t *idx, uint32_t base, uint64_t n, uint32_t *out, hipStream_t s);
-// Out-of-core host streams: index arrays packed to 24 bits (groups of four
-// indices in three words), unpacked on the device per staged segment; the
-// row index of the mode-n stream is not shipped but rebuilt from row_ptr.
+// Out-of-core host streams: index arrays packed to 16 bits (two per word)
+// or 24 bits (four in three words), unpacked on the device per staged
+// segment; the row index of the mode-n stream is not shipped but rebuilt
+// from row_ptr.  The unpackers write 4*ceil(n/4) entries.
+void pack16(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s);
+void unpack16(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s);
 void pack24(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s);
-void unpack24(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s); // writes 4*ceil(n/4)
+void unpack24(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s);
 void rows_from_ptr(const uint64_t *row_ptr, uint32_t rows, uint64_t lo, uint64_t len,
                    uint32_t *out, hipStream_t s);
 

@@ -134,6 +134,26 @@ __global__ void k_pack24(const uint32_t *__restrict__ in, uint64_t n, uint32_t *
   }
 }
 
+// 16-bit packing (indices < 2^16): two indices per word, the first in the
+// low half; a short last pair is padded with zero.
+__global__ void k_pack16(const uint32_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ out) {
+  const uint64_t pairs = (n + 1) / 2;
+  for (uint64_t pi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pi < pairs;
+       pi += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t lo = in[2 * pi], hi = 2 * pi + 1 < n ? in[2 * pi + 1] : 0u;
+    out[pi] = lo | (hi << 16);
+  }
+}
+
+__global__ void k_unpack16(const uint32_t *__restrict__ in, uint64_t quads,
+                           uint32_t *__restrict__ out) {
+  for (uint64_t qi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; qi < quads;
+       qi += (uint64_t)gridDim.x * blockDim.x) {
+    const uint2 w = reinterpret_cast<const uint2 *>(in)[qi];
+    reinterpret_cast<uint4 *>(out)[qi] = uint4{w.x & 0xFFFFu, w.x >> 16, w.y & 0xFFFFu, w.y >> 16};
+  }
+}
+
 __global__ void k_unpack24(const uint32_t *__restrict__ in, uint64_t groups,
                            uint32_t *__restrict__ out) {
   for (uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < groups;
@@ -238,6 +258,19 @@ void sub_u32(const uint32_t *idx, uint32_t base, uint64_t n, uint32_t *out, hipS
 void pack24(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_pack24, dim3(grid_for((n + 3) / 4, 256)), dim3(256), 0, s, in, n, out);
+  GT_HIP(hipGetLastError());
+}
+
+void pack16(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_pack16, dim3(grid_for((n + 1) / 2, 256)), dim3(256), 0, s, in, n, out);
+  GT_HIP(hipGetLastError());
+}
+
+void unpack16(const uint32_t *in, uint64_t n, uint32_t *out, hipStream_t s) {
+  if (!n) return;
+  const uint64_t quads = (n + 3) / 4;
+  hipLaunchKernelGGL(k_unpack16, dim3(grid_for(quads, 256)), dim3(256), 0, s, in, quads, out);
   GT_HIP(hipGetLastError());
 }
 

@@ -79,9 +79,10 @@ typedef struct gt_partition_info {
   uint64_t avail_nnz_per_task;
   int out_of_core;           /* 1: streams staged from pinned host memory per mode update */
   uint64_t segment_nnz;      /* nonzeros per staged segment (out-of-core) */
-  uint32_t stream_bytes_per_nnz; /* out-of-core: host bytes per nonzero shipped per mode update
-                                    (24-bit packed indices without the row index when every
-                                    factor has at most 2^24 rows, else 4 bytes per index) */
+  uint32_t stream_bytes_per_nnz; /* out-of-core: host bytes per nonzero shipped per mode update,
+                                    the largest over the modes (indices packed to 16 or 24 bits
+                                    and no row index when every factor has at most 2^24 rows,
+                                    else 4 bytes per index) */
 } gt_partition_info;
 
 typedef struct gt_timings {

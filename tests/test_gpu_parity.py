@@ -222,10 +222,10 @@ def test_skewed_sample_of_benchmark_workload(prec, tol):
 def test_out_of_core_matches_in_core(prec, pack, monkeypatch):
     """A device budget below the resident layout stages the per-mode streams
     from pinned host memory segment by segment (double-buffered copies).  With
-    every factor below 2^24 rows the host streams hold 24-bit packed indices
-    and no row index (10 bytes per nonzero for order 3), unpacked and rebuilt
-    from row_ptr on the device; GT_STREAM_PACK=0 ships plain 32-bit indices
-    (16 bytes).  The kernels see the same chunks in the same order, so factors
+    every factor at most 2^24 rows the host streams hold packed indices (16
+    bits here, every factor having at most 2^16 rows) and no row index: 8
+    bytes per nonzero for order 3, unpacked and rebuilt from row_ptr on the
+    device; GT_STREAM_PACK=0 ships plain 32-bit indices (16 bytes).  The kernels see the same chunks in the same order, so factors
     must be bit-identical to the in-core run; the fit sums its partials in
     another grouping, so it agrees to FP64 rounding."""
     monkeypatch.setenv("GT_STREAM_PACK", "1" if pack else "0")
@@ -241,7 +241,7 @@ def test_out_of_core_matches_in_core(prec, pack, monkeypatch):
     oc.set_init(A, G)
     info = oc.info()
     assert info["out_of_core"] == 1
-    assert info["stream_bytes_per_nnz"] == (10 if pack else 16)
+    assert info["stream_bytes_per_nnz"] == (8 if pack else 16)
     assert 0 < info["segment_nnz"] < 30000 and info["segment_nnz"] % 64 == 0
     f_ref, f_oc = ref.run(), oc.run()
     for m in range(3):
@@ -315,7 +315,7 @@ def test_order5_out_of_core_sample(prec):
     oc = make(idx, vals, dims, 5, [1, 1, 1, 1, 1], device_budget=2_000_000, **kw)
     oc.set_init(A, G)
     assert oc.info()["out_of_core"] == 1 and ref.info()["out_of_core"] == 0
-    assert oc.info()["stream_bytes_per_nnz"] == 3 * 4 + 4  # four packed indices and the value
+    assert oc.info()["stream_bytes_per_nnz"] == 2 * 4 + 4  # four 16-bit indices and the value
     f_ref, f_oc = ref.run(), oc.run()
     for m in range(5):
         assert np.array_equal(ref.get_factor(m), oc.get_factor(m)), m
