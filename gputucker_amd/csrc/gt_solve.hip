@@ -28,47 +28,62 @@ __host__ __device__ constexpr int up_index(int R, int p, int q) {
 template <int R>
 __global__ void __launch_bounds__(64) k_solve_lane(SolveArgs a) {
   constexpr int W = gram_width(R);
-  constexpr int kPiece = 16; // rows staged through LDS at a time
-  __shared__ __align__(16) double buf[kPiece * W];
+  constexpr int kPiece = 16;                                   // rows per staged piece
+  constexpr int kPieceD2 = kPiece * W / 2;                     // 16-byte units per piece
+  constexpr int kCopy = (kPieceD2 + kWave - 1) / kWave;        // load instructions per piece
+  __shared__ __align__(16) double buf[2][kCopy * kWave * 2];   // two pieces, whole instructions
   const int lane = threadIdx.x;
   const uint64_t row0 = (uint64_t)blockIdx.x * kWave;
   const uint64_t row = row0 + lane;
   // The wave's 64 Gram rows are contiguous.  They are read 16 rows at a time
   // as one block of 16-byte loads (whole cache lines, every byte used once)
-  // into LDS, and the 16 lanes that own those rows copy theirs out; a
-  // lane-per-row read would touch 64 lines per load instruction and refetch
-  // them as they are evicted.
-  // the row's degree is read first, so its latency overlaps the staging
-  const bool empty = row < a.rows && a.row_ptr[row + 1] == a.row_ptr[row];
+  // straight into LDS (global_load_lds: no registers held while in flight),
+  // two pieces in flight, and the 16 lanes that own a piece's rows copy
+  // theirs out.  A lane-per-row read would touch 64 lines per load
+  // instruction and refetch them as they are evicted.
+  // the row's degree is loaded first (unconditionally, from an in-bounds row)
+  // and compared after the staging, so its latency overlaps the staging
+  const uint64_t rc = row < a.rows ? row : 0;
+  const uint64_t rp_lo = a.row_ptr[rc], rp_hi = a.row_ptr[rc + 1];
+  auto rows_in = [&](int k) -> int {
+    const uint64_t pr = row0 + (uint64_t)k * kPiece;
+    return pr < a.rows ? (int)(a.rows - pr < (uint64_t)kPiece ? a.rows - pr : kPiece) : 0;
+  };
+  // A piece with rows issues exactly kCopy loads (lanes past its end reload
+  // its last 16 bytes into LDS slots nobody reads), so the wait for piece k
+  // while piece k + 1 is in flight is a fixed count.
+  auto issue = [&](int k) {
+    const int n2 = rows_in(k) * W / 2;
+    if (n2 == 0) return;
+    const double2 *src = reinterpret_cast<const double2 *>(a.gram + (row0 + (uint64_t)k * kPiece) * W);
+#pragma unroll
+    for (int c = 0; c < kCopy; ++c) {
+      // lane l of copy c lands at byte 1024 c + 16 l of the buffer
+      const int i = lane + c * kWave < n2 ? lane + c * kWave : n2 - 1;
+      __builtin_amdgcn_global_load_lds(
+          src + i, (__attribute__((address_space(3))) void *)(buf[k & 1] + c * kWave * 2), 16, 0,
+          0);
+    }
+  };
+  // s_waitcnt immediates: vmcnt(kCopy) or vmcnt(0), other counters untouched
+  constexpr int kWaitNext = (kCopy & 0xF) | ((kCopy >> 4) << 14) | (7 << 4) | (15 << 8);
+  constexpr int kWaitAll = (7 << 4) | (15 << 8);
+  static_assert(kCopy < 64, "vmcnt holds at most 63");
   double L[R * (R + 1) / 2]; // packed lower: L(i, j) at i(i+1)/2 + j
   double cv[R];
+  issue(0);
+  issue(1);
 #pragma unroll
   for (int k = 0; k < kWave / kPiece; ++k) {
-    const uint64_t pr = row0 + (uint64_t)k * kPiece;
-    const int n = pr < a.rows ? (int)(a.rows - pr < (uint64_t)kPiece ? a.rows - pr : kPiece) : 0;
-    if (n > 0) { // the piece in two batches of loads in flight, each then written to LDS
-      constexpr int kCopy = (kPiece * W / 2 + kWave - 1) / kWave, kHalf = (kCopy + 1) / 2;
-      const double2 *src = reinterpret_cast<const double2 *>(a.gram + pr * W);
-      double2 *dst = reinterpret_cast<double2 *>(buf);
-      const int n2 = n * W / 2;
-#pragma unroll
-      for (int h = 0; h < kCopy; h += kHalf) {
-        double2 t[kHalf];
-#pragma unroll
-        for (int c = 0; c < kHalf; ++c) {
-          const int i = lane + (h + c) * kWave;
-          t[c] = (h + c < kCopy && i < n2) ? src[i] : double2{0.0, 0.0};
-        }
-#pragma unroll
-        for (int c = 0; c < kHalf; ++c) {
-          const int i = lane + (h + c) * kWave;
-          if (h + c < kCopy && i < n2) dst[i] = t[c];
-        }
-      }
-    }
+    const int n = rows_in(k);
+    // piece k has landed: at most the next piece's loads are still in flight
+    if (k + 1 < kWave / kPiece && rows_in(k + 1) > 0)
+      __builtin_amdgcn_s_waitcnt(kWaitNext);
+    else
+      __builtin_amdgcn_s_waitcnt(kWaitAll);
     __syncthreads();
     if (lane / kPiece == k && lane % kPiece < n) {
-      const double *g = buf + (lane % kPiece) * W;
+      const double *g = buf[k & 1] + (lane % kPiece) * W;
 #pragma unroll
       for (int i = 0; i < R; ++i)
 #pragma unroll
@@ -77,7 +92,9 @@ __global__ void __launch_bounds__(64) k_solve_lane(SolveArgs a) {
       for (int i = 0; i < R; ++i) cv[i] = g[R * (R + 1) / 2 + i];
     }
     __syncthreads();
+    if (k + 2 < kWave / kPiece) issue(k + 2);
   }
+  const bool empty = rp_hi == rp_lo;
   double contrib = 0.0;
   if (row < a.rows) {
     float *out = a.A + row * row_stride(R);
