@@ -161,4 +161,38 @@ contract_slots(int slots, const float *__restrict__ A_L, uint32_t iL,
   }
 }
 
+// contract_slots for N = 3 (one outer slot), R <= 16, on two nonzeros per
+// lane: every core value read through a scalar load feeds two FMAs, so the
+// wave waits on half as many scalar loads per nonzero.  Each delta is formed
+// in the same order as contract_slots forms it.
+typedef __attribute__((address_space(4))) const float gt_cfloat;
+template <int R>
+__device__ inline __attribute__((always_inline)) void
+contract_pair(const float *__restrict__ A_L, const float *__restrict__ A_O,
+              const uint32_t (&iL)[2], const uint32_t (&io)[2], gt_cfloat *__restrict__ Gt,
+              float (&d0)[R], float (&d1)[R]) {
+  static_assert(R <= 16, "register contraction covers R <= 16");
+  constexpr int RS = row_stride(R);
+  float aL0[RS], aO0[RS], aL1[RS], aO1[RS];
+  load_row<R>(A_L, iL[0], aL0);
+  load_row<R>(A_O, io[0], aO0);
+  load_row<R>(A_L, iL[1], aL1);
+  load_row<R>(A_O, io[1], aO1);
+#pragma unroll
+  for (int r = 0; r < R; ++r) d0[r] = d1[r] = 0.f;
+#pragma unroll
+  for (int o = 0; o < R; ++o) {
+#pragma unroll
+    for (int jL = 0; jL < R; ++jL) {
+      const float K0 = aO0[o] * aL0[jL], K1 = aO1[o] * aL1[jL];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float g = Gt[(o * R + jL) * R + r];
+        d0[r] = __builtin_fmaf(g, K0, d0[r]);
+        d1[r] = __builtin_fmaf(g, K1, d1[r]);
+      }
+    }
+  }
+}
+
 } // namespace gt

@@ -230,18 +230,70 @@ template <int R>
 __global__ void __launch_bounds__(256) k_gram_valu(GramArgs a, int wpb) {
   static_assert(R <= 15, "MFMA Gram covers R <= 15");
   constexpr int T = ChunkGram<R>::T;
-  __shared__ float s_tile[4][16 * kDP];
-  __shared__ uint32_t s_row[4][kWave];
+  __shared__ float s_tile[4][2][16 * kDP];
+  __shared__ uint32_t s_row[4][2][kWave];
   extern __shared__ __align__(16) float dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const int slots = a.slots;
   const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
   if (c >= a.chunk_end) return;
   float *lrows = dyn + (size_t)w * lds_floats_per_wave<kValu>(slots, R, 1);
-  float *tile = s_tile[w];
+  float *tile = s_tile[w][0];
   const uint64_t cb = c * a.chunk;
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
   ChunkGram<R> gram(a, c, cb, ce, lane);
+
+  if constexpr (R <= 16) {
+    if (slots == 1) { // N = 3: two tiles per delta pass (contract_pair)
+      uint32_t p_row[2], p_iL[2], p_io[2];
+      float p_x[2];
+      auto load_pair = [&](uint64_t tb) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const uint64_t e = tb + h * kWave + lane;
+          const uint64_t ee = e < ce ? e : cb;
+          p_row[h] = a.idx_row[ee];
+          p_iL[h] = a.idx_L[ee];
+          p_io[h] = a.idx_O[0][ee];
+          p_x[h] = e < ce ? a.st.val[ee] : 0.f;
+        }
+      };
+      load_pair(cb);
+      for (uint64_t tb = cb; tb < ce; tb += 2 * kWave) {
+        const uint32_t row[2] = {p_row[0], p_row[1]};
+        const uint32_t iL[2] = {p_iL[0], p_iL[1]}, io[2] = {p_io[0], p_io[1]};
+        const float x[2] = {p_x[0], p_x[1]};
+        if (tb + 2 * kWave < ce) load_pair(tb + 2 * kWave);
+        float d[2][R];
+        // the core is read through the constant address space with a zero
+        // offset the compiler cannot see through: its loads stay scalar and
+        // inside the loop (hoisted, its 4R^3 bytes would sit in vector
+        // registers)
+        uint32_t goff = 0;
+        asm volatile("" : "+s"(goff));
+        contract_pair<R>(a.A_L, a.A_O[0], iL, io, (const gt_cfloat *)(a.Gt + goff), d[0], d[1]);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const bool valid = tb + h * kWave + lane < ce;
+          s_row[w][h][lane] = valid ? row[h] : 0xFFFFFFFFu;
+#pragma unroll
+          for (int r = 0; r < 4 * T; ++r)
+            s_tile[w][h][r * kDP + lane] = (r < R) ? (valid ? d[h][r] : 0.f) : (r == R ? x[h] : 0.f);
+        }
+        wave_sync();
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+          const uint64_t th = tb + h * kWave;
+          if (th >= ce) break;
+          const int cnt = (ce - th < (uint64_t)kWave) ? (int)(ce - th) : kWave;
+          gram.add_tile(s_tile[w][h], s_row[w][h], th, cnt);
+        }
+        wave_sync();
+      }
+      gram.finish();
+      return;
+    }
+  }
 
   // stream indices of the next tile are prefetched behind the current
   // tile's gathers, so each tile waits for one memory round trip, not three
@@ -265,7 +317,7 @@ __global__ void __launch_bounds__(256) k_gram_valu(GramArgs a, int wpb) {
 #pragma unroll
     for (int t = 0; t < kSlots; ++t) io[t] = n_io[t];
     const float x = n_x;
-    s_row[w][lane] = valid ? row : 0xFFFFFFFFu;
+    s_row[w][0][lane] = valid ? row : 0xFFFFFFFFu;
     float d[R];
     if (tb + kWave < ce) load_idx(tb + kWave);
     contract_slots<R>(slots, a.A_L, iL, a.A_O, io, a.Gt, lrows, lane, d);
@@ -273,7 +325,7 @@ __global__ void __launch_bounds__(256) k_gram_valu(GramArgs a, int wpb) {
     for (int r = 0; r < 4 * T; ++r)
       tile[r * kDP + lane] = (r < R) ? (valid ? d[r] : 0.f) : (r == R ? x : 0.f);
     wave_sync();
-    gram.add_tile(tile, s_row[w], tb, cnt);
+    gram.add_tile(tile, s_row[w][0], tb, cnt);
     wave_sync();
   }
   gram.finish();
