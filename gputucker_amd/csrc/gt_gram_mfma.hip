@@ -16,11 +16,11 @@
 // BF16 delta, two stages per 64-nonzero tile:
 //   stage 1 (MFMA):      U[m][(o,r)] = sum_jL A_L[i_L(m)][jL] G[o][jL][r]
 //   stage 2 (VALU FP32): delta[m][r] = sum_o U[m][(o,r)] * s_m[o]
-// with s_m[o] = prod_t A_t[i_t(m)][o_t].  The A operand of stage 1 is read
-// ready-made from a split copy of A_L (k_split_factor: per row and lane
-// group the 16 bytes of BF16 words the group's K=32 slots take, see
-// bf16r_plan), so a tile spends no arithmetic on it; the B operand (the core)
-// comes from a small table in LDS laid out in lane order.  The gathered
+// with s_m[o] = prod_t A_t[i_t(m)][o_t].  The nonzero-side operand of stage
+// 1 is read ready-made from a split copy of A_L (k_split_factor: per row and
+// lane group the 16 bytes of BF16 words the group's K=32 slots take, see
+// bf16r_plan), so a tile spends no arithmetic on it; the core operand comes
+// from a small table in LDS laid out in lane order.  The gathered
 // operands of the next tile and the stream indices of the tile after it are
 // loaded while the current tile is computed.
 //
@@ -423,14 +423,19 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
     u4 a32[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) a32[mb] = g32[mb];
-    float aO[RS];
-#pragma unroll
-    for (int j = 0; j < RS; ++j) aO[j] = gO[j];
     const uint32_t row = c_row;
     const float x = c_x;
     uint32_t io[kSlots];
 #pragma unroll
     for (int t = 0; t < kSlots; ++t) io[t] = c_io[t];
+    s_row[w][lane] = valid ? row : 0xFFFFFFFFu;
+    // N = 3: s_m[o] is this lane's outer row, written to LDS before the
+    // next gather reuses its registers (padding lanes gathered in-bounds
+    // rows; their deltas never enter the Gram)
+    if (slots == 1) {
+#pragma unroll
+      for (int o = 0; o < R; ++o) souter[o * kWave + lane] = gO[o];
+    }
     c_row = n_row;
     c_x = n_x;
 #pragma unroll
@@ -438,16 +443,10 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
     gather(n_iL, n_io[0]);
     load_idx(tb + 2 * kWave);
 
-    s_row[w][lane] = valid ? row : 0xFFFFFFFFu;
     // s_m[o] for this lane's nonzero, into LDS for the stage-2 layout
     if (slots == 0) {
       souter[lane] = valid ? 1.f : 0.f;
-    } else if (slots == 1) {
-      // padding lanes gathered in-bounds rows; their deltas never enter the
-      // Gram (the runs end at the tile's last valid nonzero)
-#pragma unroll
-      for (int o = 0; o < R; ++o) souter[o * kWave + lane] = aO[o];
-    } else {
+    } else if (slots >= 2) {
 #pragma unroll
       for (int t = 0; t < kSlots; ++t) {
         if (t >= slots) continue;
@@ -480,6 +479,10 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
+    // The core table is the A operand and the split A_L words the B
+    // operand, so each MFMA yields U transposed: lane (q, g) holds
+    // U[16mb + q][o][4g + i], i < 4, and stage 2 needs one s value per
+    // block and o (s_m[o] of the lane's own column nonzero) for four FMAs.
     auto stage = [&](int o) {
       f4 u[4];
       const f4 z = {0.f, 0.f, 0.f, 0.f};
@@ -488,26 +491,26 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb)
           u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-              __builtin_bit_cast(s4, u2{a32[mb][0], a32[mb][1]}), bh, z, 0, 0, 0);
+              bh, __builtin_bit_cast(s4, u2{a32[mb][0], a32[mb][1]}), z, 0, 0, 0);
       } else {
         const b8 b = __builtin_bit_cast(b8, t32[o * kWave + lane]);
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb)
-          u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a32[mb]), b, z, 0,
+          u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, a32[mb]), z, 0,
                                                           0, 0);
         if constexpr (!bf16r_single_mfma(R)) {
           const s4 bl = t16lo[o * kWave + lane];
 #pragma unroll
           for (int mb = 0; mb < 4; ++mb)
             u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                __builtin_bit_cast(s4, u2{a32[mb][0], a32[mb][1]}), bl, u[mb], 0, 0, 0);
+                bl, __builtin_bit_cast(s4, u2{a32[mb][0], a32[mb][1]}), u[mb], 0, 0, 0);
         }
       }
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
-        const f4 sv = *reinterpret_cast<const f4 *>(&souter[o * kWave + mb * 16 + 4 * g]);
+        const float sv = souter[o * kWave + mb * 16 + q];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv[i], dacc[mb][i]);
+        for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
       }
     };
     if (slots == 0) {
@@ -518,13 +521,13 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
     } else {
       for (int o = 0; o < O; ++o) stage(o);
     }
-    // lane (q, g) holds delta[16mb + 4g + i][q]: columns to LDS, then x
+    // lane (q, g) holds delta[16mb + q][4g + i]: columns to LDS, then x
     // (zero for padding lanes) over column R
-    if (q < 4 * T) {
+    if (g < T) {
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
-        *reinterpret_cast<f4 *>(&tile[q * kDP + mb * 16 + 4 * g]) =
-            f4{dacc[mb][0], dacc[mb][1], dacc[mb][2], dacc[mb][3]};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) tile[(4 * g + i) * kDP + mb * 16 + q] = dacc[mb][i];
     }
     tile[R * kDP + lane] = x;
     wave_sync();
