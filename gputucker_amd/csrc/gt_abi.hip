@@ -377,18 +377,14 @@ float from_bf16(uint16_t h) {
   return x;
 }
 
-// B operand of the 16x16x16 BF16 MFMA for every outer index o, in lane
-// order: lane l, element t holds G[o][jL = 4(l/16)+t][r = l%16] (zero outside
-// R x R).  The hi table (BF16 of G) is followed by the lo table (BF16 of the
-// rounding remainder G - hi).
 // BF16 core operand tables of the MFMA delta (layout and plans in
-// gt_internal.hpp).  Entry (o, lane, t) of a table is the B operand word of
-// K slot t for output column r = lane % 16 of outer index o; G[o][jL][r] is
-// zero outside jL, r < R.
+// gt_internal.hpp).  Entry (o, lane, t) of a table is the core operand word
+// of K slot t for output r = lane % 16 of outer index o; G[o][jL][r] is zero
+// outside jL, r < R.
 std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R) {
   const int O = (int)ipow(R, N > 2 ? N - 2 : 0);
   const size_t n16 = gb_k16_words(O), n32 = gb_k32_words(O);
-  std::vector<uint16_t> tab(2 * n16 + n32, 0);
+  std::vector<uint16_t> tab(n16 + 2 * n32, 0);
   auto g_at = [&](int o, int jL, int r) -> float {
     return (jL < R && r < R) ? gt[((size_t)o * R + jL) * R + r] : 0.f;
   };
@@ -400,7 +396,7 @@ std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R
       for (int t = 0; t < 4; ++t) {
         const size_t at = ((size_t)o * kWave + l) * 4 + t;
         tab[at] = hi(g_at(o, 4 * g + t, r));                 // K16 hi
-        tab[n16 + n32 + at] = lo(g_at(o, 4 * g + t, r));     // K16 lo
+        tab[n16 + n32 + ((size_t)o * kWave + l) * 8 + t] = lo(g_at(o, 4 * g + t, r)); // K32 lo
       }
       uint16_t *k32 = tab.data() + n16 + ((size_t)o * kWave + l) * 8;
       switch (bf16r_plan(R, g)) {

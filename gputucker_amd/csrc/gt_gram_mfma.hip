@@ -72,6 +72,14 @@ template <int R> __device__ inline int packed_index(int p, int q) {
   return -1;
 }
 
+// LDS one workgroup may take (a launch without attributes), and the static
+// LDS of the Gram kernels (delta tiles and row ids of four waves; the FP32
+// kernel keeps two of each per wave), which the dynamic part must leave room
+// for.
+constexpr size_t kLdsLimit = 64 * 1024;
+constexpr size_t kStaticLdsValu = 4 * 2 * (16 * kDP * sizeof(float) + kWave * sizeof(uint32_t));
+constexpr size_t kStaticLdsBf16 = 4 * (16 * kDP * sizeof(float) + kWave * sizeof(uint32_t));
+
 template <int KIND> __host__ __device__ constexpr size_t lds_floats_per_wave(int slots, int R,
                                                                              int O) {
   return (slots >= 2 ? (size_t)slots * R * kWave : 0) + (KIND == kValu ? 0 : (size_t)O * kWave);
@@ -348,11 +356,11 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const int slots = a.slots;
   // the core operand tables go to LDS once per workgroup:
-  // kBf16: K16 hi table; kBf16R: K32 table, then (R > 10) the K16 lo table
+  // kBf16: K16 hi table; kBf16R: K32 table, then (R > 10) the K32 lo table
   uint4 *s_tab = reinterpret_cast<uint4 *>(dyn + (size_t)wpb * lds_floats_per_wave<KIND>(slots, R, O));
   const s4 *t16 = reinterpret_cast<const s4 *>(s_tab);
   const s8 *t32 = reinterpret_cast<const s8 *>(s_tab);
-  const s4 *t16lo = reinterpret_cast<const s4 *>(s_tab + (size_t)O * kWave);
+  const s8 *t32lo = reinterpret_cast<const s8 *>(s_tab + (size_t)O * kWave);
   {
     const uint4 *src = reinterpret_cast<const uint4 *>(a.Gb);
     if constexpr (KIND == kBf16) {
@@ -364,7 +372,7 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
       for (int i = threadIdx.x; i < n; i += blockDim.x) s_tab[i] = k32[i];
       if constexpr (!bf16r_single_mfma(R)) {
         const uint4 *lo = k32 + n;
-        const int n2 = (int)(gb_k16_words(O) / 8);
+        const int n2 = (int)(gb_k32_words(O) / 8);
         for (int i = threadIdx.x; i < n2; i += blockDim.x) s_tab[n + i] = lo[i];
       }
     }
@@ -499,11 +507,11 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
           u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, a32[mb]), z, 0,
                                                           0, 0);
         if constexpr (!bf16r_single_mfma(R)) {
-          const s4 bl = t16lo[o * kWave + lane];
+          const b8 bl = __builtin_bit_cast(b8, t32lo[o * kWave + lane]);
 #pragma unroll
           for (int mb = 0; mb < 4; ++mb)
-            u[mb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(
-                bl, __builtin_bit_cast(s4, u2{a32[mb][0], a32[mb][1]}), u[mb], 0, 0, 0);
+            u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bl, __builtin_bit_cast(b8, a32[mb]),
+                                                            u[mb], 0, 0, 0);
         }
       }
 #pragma unroll
@@ -560,7 +568,9 @@ __global__ void __launch_bounds__(256) k_split_factor(const float *__restrict__ 
 template <int R> void launch_valu(const GramArgs &a, hipStream_t st) {
   const size_t per_wave = lds_floats_per_wave<kValu>(a.slots, R, 1) * sizeof(float);
   int wpb = 4;
-  while (wpb > 1 && per_wave * wpb > 64 * 1024) --wpb;
+  while (wpb > 1 && per_wave * wpb + kStaticLdsValu > kLdsLimit) --wpb;
+  if (per_wave * wpb + kStaticLdsValu > kLdsLimit)
+    throw std::runtime_error("FP32 Gram: outer factor rows do not fit in LDS");
   const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
   hipLaunchKernelGGL((k_gram_valu<R>), dim3(grid), dim3(64 * wpb), per_wave * wpb, st, a, wpb);
 }
@@ -573,13 +583,13 @@ template <int R, int KIND> void launch_impl(const GramArgs &a, int O, hipStream_
     const size_t per_wave = lds_floats_per_wave<KIND>(a.slots, R, O) * sizeof(float);
     const size_t table =
         KIND == kBf16 ? gb_k16_words(O) * 2
-                      : (gb_k32_words(O) + (bf16r_single_mfma(R) ? 0 : gb_k16_words(O))) * 2;
-    if (per_wave + table > 64 * 1024 || a.As_L == nullptr) { // core tables too large for LDS
+                      : (gb_k32_words(O) * (bf16r_single_mfma(R) ? 1 : 2)) * 2;
+    if (per_wave + table + kStaticLdsBf16 > kLdsLimit || a.As_L == nullptr) { // tables too large
       launch_valu<R>(a, st);
       return;
     }
     int wpb = 4;
-    while (wpb > 1 && per_wave * wpb + table > 64 * 1024) --wpb;
+    while (wpb > 1 && per_wave * wpb + table + kStaticLdsBf16 > kLdsLimit) --wpb;
     const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
     hipLaunchKernelGGL((k_gram_bf16<R, KIND>), dim3(grid), dim3(64 * wpb),
                        per_wave * wpb + table, st, a, wpb, O);

@@ -176,8 +176,11 @@ bool gram_mfma_supported(int R);
 //                                   x [Ghi[8..11], Ghi[8], Ghi[9], Glo[8], Glo[9]]
 //   kPlanZero zeros
 // R <= 10 fits all three terms in the 32 slots; above that the K=32 MFMA
-// carries the first two terms (kPlanHL everywhere) and a 16x16x16 MFMA adds
-// a_hi.G_lo.
+// carries the first two terms (kPlanHL everywhere) and a second K=32 MFMA,
+// accumulating into the first one's result, adds a_hi.G_lo: its core words
+// are [Glo[4g..4g+3], 0, 0, 0, 0], so the lo words of the factor row meet
+// zeros.  Both MFMAs are the same instruction, so the second may depend on
+// the first back to back.
 enum Bf16rPlan { kPlanZero = 0, kPlanHL = 1, kPlanHH = 2, kPlanR10 = 3 };
 __host__ __device__ constexpr bool bf16r_single_mfma(int R) { return R <= 10; }
 __host__ __device__ constexpr int bf16r_plan(int R, int g) {
@@ -187,8 +190,8 @@ __host__ __device__ constexpr int bf16r_plan(int R, int g) {
                  : (g == 0 ? kPlanHL : (g == 1 ? kPlanHH : kPlanZero));
 }
 // BF16 core operand tables, in uint16 words: K16 hi table ([o][lane][4]),
-// then the K32 refined table ([o][lane][8]), then the K16 lo table
-// ([o][lane][4]; used when R > 10).
+// then the K32 refined table ([o][lane][8]), then the K32 lo table
+// ([o][lane][8]; used when R > 10).
 __host__ __device__ constexpr size_t gb_k16_words(int O) { return (size_t)O * kWave * 4; }
 __host__ __device__ constexpr size_t gb_k32_words(int O) { return (size_t)O * kWave * 8; }
 void launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
