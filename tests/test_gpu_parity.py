@@ -167,7 +167,8 @@ def test_error_paths():
 
 @pytest.mark.parametrize("prec,tol_gram", [("bf16", 3e-2), ("bf16r", 2e-4), ("fp32", 1e-5)])
 @pytest.mark.parametrize("N,R,part", [(3, 10, [2, 1, 2]), (4, 5, [1, 2, 1, 1]), (2, 8, [1, 3]),
-                                      (3, 13, [2, 1, 2]), (3, 15, [1, 1, 1]), (3, 7, [1, 2, 1])])
+                                      (3, 13, [2, 1, 2]), (3, 15, [1, 1, 1]), (3, 7, [1, 2, 1]),
+                                      (2, 13, [2, 1])])
 def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
     """BF16 MFMA delta (plain and with the split-BF16 refinement; above
     R = 10 the refinement takes a second MFMA per tile) and the FP32 VALU
@@ -185,6 +186,26 @@ def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
         assert (np.abs(Bg - B) / sc).max() < tol_gram, (m, (np.abs(Bg - B) / sc).max())
         scc = np.abs(Cv).max(axis=1, keepdims=True) + 1e-30
         assert (np.abs(Cg - Cv) / scc).max() < tol_gram
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16r"])
+@pytest.mark.parametrize("N,R,part", [(4, 15, [1, 2, 1, 1]), (5, 12, [1, 1, 2, 1, 1])])
+def test_gram_parity_high_order_wide_rank(prec, N, R, part):
+    """Orders 4 and 5 at R = 15 and 12: the BF16 core tables (R^(N-2) outer
+    indices) do not fit in LDS, so both precisions take the FP32 VALU delta,
+    whose outer factor rows staged in LDS leave room for three and two waves
+    per workgroup."""
+    dims = [40, 30, 25, 20, 15][:N]
+    e, t, A, G = _setup(N * 7 + R, dims, 1500, R, part, precision=prec)
+    lam = gt.default_lambda()
+    for m in range(N):
+        d = oracle.delta(t, A, G, m)
+        B, Cv = oracle.bc(t, d, m, lam)
+        Bg, Cg = e.get_gram(m, t.rows(m))
+        sc = np.abs(B).max(axis=(1, 2), keepdims=True)
+        assert (np.abs(Bg - B) / sc).max() < 1e-5, (m, (np.abs(Bg - B) / sc).max())
+        scc = np.abs(Cv).max(axis=1, keepdims=True) + 1e-30
+        assert (np.abs(Cg - Cv) / scc).max() < 1e-5
 
 
 @pytest.mark.parametrize("prec,tol", [("bf16r", 1e-4), ("bf16", 2e-3), ("fp32", 1e-5)])
