@@ -143,18 +143,37 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
 // order, so results are deterministic.
 constexpr int kPiece = 64; // chunks per first-level piece (one per lane)
 
-__device__ inline double sum_bpart0(const double *bp, uint64_t first, uint64_t last, uint64_t step,
-                                    int W, int q) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0; // four independent chains
+// out[j] = sum of bp[k][lane + 64 j] over k = first, first + step, ..., <= last,
+// for every entry lane + 64 j < NE: all of the lane's entries in one pass,
+// four independent chains each, so a wave waits for one memory round trip per
+// four chunks however many entries a row has
+template <int NE>
+__device__ inline void sum_bpart0(const double *bp, uint64_t first, uint64_t last, uint64_t step,
+                                  int W, int lane, double (&out)[(NE + kWave - 1) / kWave]) {
+  constexpr int NQ = (NE + kWave - 1) / kWave;
+  double s[NQ][4];
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) s[j][0] = s[j][1] = s[j][2] = s[j][3] = 0.0;
   uint64_t k = first;
   for (; k + 3 * step <= last; k += 4 * step) {
-    s0 += bp[k * W + q];
-    s1 += bp[(k + step) * W + q];
-    s2 += bp[(k + 2 * step) * W + q];
-    s3 += bp[(k + 3 * step) * W + q];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      const int q = lane + kWave * j;
+      if (q < NE) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) s[j][u] += bp[(k + u * step) * W + q];
+      }
+    }
   }
-  for (; k <= last; k += step) s0 += bp[k * W + q];
-  return (s0 + s1) + (s2 + s3);
+  for (; k <= last; k += step) {
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+      const int q = lane + kWave * j;
+      if (q < NE) s[j][0] += bp[k * W + q];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NQ; ++j) out[j] = (s[j][0] + s[j][1]) + (s[j][2] + s[j][3]);
 }
 
 // Each fix-up workgroup has kFixWaves waves.  All waves find the same row
@@ -167,7 +186,7 @@ constexpr uint64_t kLongRow = 8; // pieces above which all waves share a row
 
 template <int R>
 __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_pieces(GramArgs a) {
-  constexpr int W = gram_width(R), NE = gram_entries(R);
+  constexpr int W = gram_width(R), NE = gram_entries(R), NQ = (NE + kWave - 1) / kWave;
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
   const uint32_t none = 0xFFFFFFFFu;
@@ -186,14 +205,17 @@ __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_pieces(GramArgs 
     if (e == j) continue;
     const uint64_t cj = (uint64_t)blockIdx.x * kPiece + j;
     double *first = (a.row_ptr[r] <= cj * a.chunk ? a.bpart0 : a.bpart1) + cj * W;
-    for (int q = lane; q < NE; q += kWave)
-      first[q] += sum_bpart0(a.bpart0, cj + 1, cj + (e - j), 1, W, q);
+    double sum[NQ];
+    sum_bpart0<NE>(a.bpart0, cj + 1, cj + (e - j), 1, W, lane, sum);
+#pragma unroll
+    for (int jq = 0; jq < NQ; ++jq)
+      if (lane + kWave * jq < NE) first[lane + kWave * jq] += sum[jq];
   }
 }
 
 template <int R>
 __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_rows(GramArgs a) {
-  constexpr int W = gram_width(R), NE = gram_entries(R);
+  constexpr int W = gram_width(R), NE = gram_entries(R), NQ = (NE + kWave - 1) / kWave;
   __shared__ double red[kFixWaves][NE];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = (uint64_t)blockIdx.x * kPiece + lane;
@@ -212,9 +234,12 @@ __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_rows(GramArgs a)
     const uint64_t p0 = cj / kPiece, pl = last / kPiece;
     const double *first = (rb == cj * a.chunk ? a.bpart0 : a.bpart1) + cj * W;
     if (pl - p0 > kLongRow) { // the same decision in every wave
-      for (int q = lane; q < NE; q += kWave)
-        red[w][q] =
-            sum_bpart0(a.bpart0, (p0 + 1 + w) * kPiece, pl * kPiece, kPiece * kFixWaves, W, q);
+      double sum[NQ];
+      sum_bpart0<NE>(a.bpart0, (p0 + 1 + w) * kPiece, pl * kPiece, kPiece * kFixWaves, W, lane,
+                     sum);
+#pragma unroll
+      for (int jq = 0; jq < NQ; ++jq)
+        if (lane + kWave * jq < NE) red[w][lane + kWave * jq] = sum[jq];
       __syncthreads();
       if (w == 0) {
         for (int q = lane; q < NE; q += kWave)
@@ -224,10 +249,14 @@ __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_rows(GramArgs a)
       __syncthreads();
     } else {
       if ((k++ % kFixWaves) != w) continue;
-      for (int q = lane; q < NE; q += kWave) {
-        double s = first[q];
-        if (pl > p0) s += sum_bpart0(a.bpart0, (p0 + 1) * kPiece, pl * kPiece, kPiece, W, q);
-        a.gram[(size_t)r * W + q] = s;
+      double sum[NQ];
+#pragma unroll
+      for (int jq = 0; jq < NQ; ++jq) sum[jq] = 0.0;
+      if (pl > p0) sum_bpart0<NE>(a.bpart0, (p0 + 1) * kPiece, pl * kPiece, kPiece, W, lane, sum);
+#pragma unroll
+      for (int jq = 0; jq < NQ; ++jq) {
+        const int q = lane + kWave * jq;
+        if (q < NE) a.gram[(size_t)r * W + q] = pl > p0 ? first[q] + sum[jq] : first[q];
       }
     }
   }
