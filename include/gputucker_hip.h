@@ -60,7 +60,8 @@ typedef struct gt_opts {
   uint64_t device_budget;           /* bytes the planner may assume; 0 = free device memory.
                                        When the resident layout exceeds it, the per-mode
                                        streams are staged from pinned host memory */
-  int chunk_tiles;                  /* 64-nonzero tiles per Gram chunk; 0 = autotune */
+  int chunk_tiles;                  /* 64-nonzero tiles per Gram chunk; 0 = from the nonzero
+                                       count: about 16384 chunks, 1 to 16 tiles each */
   int profile;                      /* 1: time the Gram kernel with HIP events */
 } gt_opts;
 
