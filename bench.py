@@ -27,11 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 CONFIGS = {
-    # the workload the metric is quoted on
+    # the headline workload: its mode-update throughput is the reported value
     "c2": dict(name="3-order 1e6^3, 1e8 nnz Zipf(1.1), R=10, P=4x4x4", dims=[1_000_000] * 3,
                nnz=100_000_000, modes=[("zipf", 1.1)] * 3, values="uniform", rank=10,
                part=[4, 4, 4], iters=10, lam=None),
-    # the reference's own CPU-runnable case
+    # a small uniform case of the size the reference itself runs: 2000^3, 1e6 nonzeros
     "c1": dict(name="3-order 2000^3, 1e6 nnz uniform, R=10, P=1", dims=[2000] * 3,
                nnz=1_000_000, modes=None, values="uniform", rank=10, part=[1, 1, 1], iters=5,
                lam=1e-3),
