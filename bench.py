@@ -10,7 +10,7 @@ device synchronisations; t_mode is the median over the timed ALS iterations
 of (iteration time without the fit) / N, and value = nnz / t_mode.  The fit
 is computed after the timed region.
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c1] [--precision fp32]
+    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c4] [--precision fp32]
 """
 from __future__ import annotations
 
@@ -35,6 +35,19 @@ CONFIGS = {
     "c1": dict(name="3-order 2000^3, 1e6 nnz uniform, R=10, P=1", dims=[2000] * 3,
                nnz=1_000_000, modes=None, values="uniform", rank=10, part=[1, 1, 1], iters=5,
                lam=1e-3),
+    # order 4, three Zipf(1.2) modes and a short uniform one, lognormal values;
+    # the planner picks P
+    "c3": dict(name="4-order 2e5^3 x 1e3, 2e8 nnz Zipf(1.2) x3 + uniform, lognormal, R=8",
+               dims=[200_000] * 3 + [1_000], nnz=200_000_000,
+               modes=[("zipf", 1.2)] * 3 + ["uniform"], values="lognormal", rank=8, part=None,
+               iters=3, lam=None),
+    # R = 32 (the FP32 delta and the lane-per-entry Gram) with power-law rows
+    # in mode 0; Zipf(1.5) in every mode would need about 2e10 draws for 3e8
+    # distinct coordinates
+    "c4": dict(name="3-order 5e5 x 5e5 x 5e4, 3e8 nnz Zipf(1.5) in mode 0, R=32",
+               dims=[500_000, 500_000, 50_000], nnz=300_000_000,
+               modes=[("zipf", 1.5), "uniform", "uniform"], values="uniform", rank=32, part=None,
+               iters=2, lam=None),
 }
 
 HBM_PEAK_GBS = 8000.0
@@ -146,7 +159,7 @@ def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup
                 kernel_ms=k_ms, gram_phase_ms=statistics.mean(tm["ms_gram"]),
                 solve_ms=statistics.mean(tm["ms_solve"]),
                 launches=launches, total=total, setup_s=setup_s, fit_s=t_fit,
-                partition_ms=tm["ms_partition"])
+                partition_ms=tm["ms_partition"], part=[int(p) for p in info["part"]])
 
 
 def main(argv=None):
@@ -180,8 +193,10 @@ def main(argv=None):
     t0 = time.perf_counter()
     idx, vals, dims = synth.generate(cfg["dims"], cfg["nnz"], cfg["modes"], cfg["values"],
                                      seed=a.seed, device="cuda")
+    max_deg = [int(np.bincount(idx[k]).max()) for k in range(N)]
     print(f"[bench] generated {cfg['name']}: {idx.shape[1]} nnz in "
-          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+          f"{time.perf_counter() - t0:.1f}s, largest row degree per mode {max_deg}",
+          file=sys.stderr, flush=True)
     A, G = synth.init_factors(dims, R, seed=a.seed)
     lam = cfg["lam"] if cfg["lam"] is not None else gt.default_lambda()
 
@@ -195,7 +210,7 @@ def main(argv=None):
               f"(gram phase {r['gram_phase_ms']:.3f} ms, solve {r['solve_ms']:.3f} ms), "
               f"out of core {r['out_of_core']} (segment {r['segment_nnz']}), fit {r['fit']:.7f} "
               f"(reconstruction {r['fit_recon']:.7f}), setup {r['setup_s']:.1f}s "
-              f"(partition {r['partition_ms']:.0f} ms), reconstruction pass "
+              f"(partition {r['partition_ms']:.0f} ms, P {r['part']}), reconstruction pass "
               f"{r['fit_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
     head = runs[a.precision]
     bpn = bytes_per_nnz(N, R, 4)  # factor rows are gathered as FP32 on every path
@@ -203,7 +218,7 @@ def main(argv=None):
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(idx, vals, dims, cfg["part"], R, lam, A, G)
+        cpu = cpu_baseline(idx, vals, dims, head["part"], R, lam, A, G)
     out = {
         "metric": "nnz/s per ALS mode update",
         "value": head["value"],
@@ -218,7 +233,7 @@ def main(argv=None):
         "dtype": a.precision,
         "data": "synthetic",
         "config": {"workload": cfg["name"], "nnz": int(idx.shape[1]), "order": N, "rank": R,
-                   "partition": cfg["part"], "precision": a.precision,
+                   "partition": head["part"], "precision": a.precision,
                    "parallelism": "single GPU",
                    "out_of_core": bool(head["out_of_core"])},
         "fit": head["fit"],

@@ -57,6 +57,18 @@ __device__ inline __attribute__((always_inline)) void load_row(const float *__re
   for (int j = R; j < row_stride(R); ++j) out[j] = 0.f;
 }
 
+// The core as contract_slots and contract_pair read it: through the constant
+// address space (scalar loads), from a pointer offset by a zero the compiler
+// cannot see through.  Taken inside a caller's loop, this keeps the core's
+// loads in the loop; without it the compiler may hoist them out as
+// loop-invariant, into up to 256 vector registers (one wave per SIMD).
+typedef __attribute__((address_space(4))) const float gt_cfloat;
+__device__ inline __attribute__((always_inline)) gt_cfloat *core_operand(const float *Gt) {
+  uint32_t off = 0;
+  asm volatile("" : "+s"(off));
+  return (gt_cfloat *)(Gt + off);
+}
+
 // LDS floats per wave that contract_slots needs: the outer rows when there
 // are two or more outer slots (or any, above R = 16), plus A_L above R = 16.
 template <int R> __host__ __device__ constexpr int contract_lds_rows(int slots) {
@@ -78,9 +90,10 @@ template <int R>
 __device__ inline __attribute__((always_inline)) void
 contract_slots(int slots, const float *__restrict__ A_L, uint32_t iL,
                const float *const (&A_O)[kSlots], const uint32_t (&io)[kSlots],
-               const float *__restrict__ Gt, float *lrows, int lane, float (&d)[R]) {
+               const float *__restrict__ Gt_, float *lrows, int lane, float (&d)[R]) {
   constexpr int RS = row_stride(R);
   constexpr bool kReg = R <= 16;
+  gt_cfloat *__restrict__ Gt = core_operand(Gt_);
   float aL[kReg ? RS : 4];
   float aO[kReg ? RS : 4];
   if constexpr (kReg) {
@@ -135,7 +148,7 @@ contract_slots(int slots, const float *__restrict__ A_L, uint32_t iL,
 #pragma unroll
     for (int t = 0; t < kSlots; ++t)
       if (t < slots) s *= lrows[(t * R + dig[t]) * kWave + lane];
-    const float *__restrict__ gp = Gt + (size_t)o * R * R;
+    gt_cfloat *__restrict__ gp = Gt + (size_t)o * R * R;
     if constexpr (kReg) {
 #pragma unroll
       for (int jL = 0; jL < R; ++jL) {
@@ -161,30 +174,51 @@ contract_slots(int slots, const float *__restrict__ A_L, uint32_t iL,
   }
 }
 
-// contract_slots for N = 3 (one outer slot), R <= 16, on two nonzeros per
-// lane: every core value read through a scalar load feeds two FMAs, so the
-// wave waits on half as many scalar loads per nonzero.  Each delta is formed
-// in the same order as contract_slots forms it.
-typedef __attribute__((address_space(4))) const float gt_cfloat;
-template <int R>
+constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
+
+// Whether contract_pair's fully unrolled form is used for NS outer slots:
+// R^N multiply-adds per nonzero (N = NS + 2), twice over for the pair, must
+// stay a loop body the instruction cache holds, and the two nonzeros'
+// operands must leave room for three waves per SIMD (order 3 above R = 10
+// takes 194-256 registers).
+template <int R, int NS> constexpr bool pair_unrolled() {
+  return cpow(R, NS + 2) <= 4096 && (NS > 1 || R <= 10);
+}
+
+// contract_slots for NS outer slots (R^N <= 4096, so fully unrolled out of
+// registers) on two nonzeros per lane: every core value read through a scalar
+// load feeds two FMAs, so the wave waits on half as many scalar loads per
+// nonzero.  Each delta is formed in the same order as contract_slots forms
+// it: s = prod_t A_t[i_t][o_t] in slot order, then Gt[o][jL][r] * (s *
+// A_L[i_L][jL]) accumulated over o (slot 0 slowest), jL and r.
+template <int R, int NS>
 __device__ inline __attribute__((always_inline)) void
-contract_pair(const float *__restrict__ A_L, const float *__restrict__ A_O,
-              const uint32_t (&iL)[2], const uint32_t (&io)[2], gt_cfloat *__restrict__ Gt,
-              float (&d0)[R], float (&d1)[R]) {
-  static_assert(R <= 16, "register contraction covers R <= 16");
+contract_pair(const float *__restrict__ A_L, const float *const (&A_O)[kSlots],
+              const uint32_t (&iL)[2], const uint32_t (&io)[2][kSlots],
+              gt_cfloat *__restrict__ Gt, float (&d0)[R], float (&d1)[R]) {
+  static_assert(pair_unrolled<R, NS>(), "unrolled contraction covers R^N <= 4096");
   constexpr int RS = row_stride(R);
-  float aL0[RS], aO0[RS], aL1[RS], aO1[RS];
+  float aL0[RS], aL1[RS], aO0[NS][RS], aO1[NS][RS];
   load_row<R>(A_L, iL[0], aL0);
-  load_row<R>(A_O, io[0], aO0);
+#pragma unroll
+  for (int t = 0; t < NS; ++t) load_row<R>(A_O[t], io[0][t], aO0[t]);
   load_row<R>(A_L, iL[1], aL1);
-  load_row<R>(A_O, io[1], aO1);
+#pragma unroll
+  for (int t = 0; t < NS; ++t) load_row<R>(A_O[t], io[1][t], aO1[t]);
 #pragma unroll
   for (int r = 0; r < R; ++r) d0[r] = d1[r] = 0.f;
 #pragma unroll
-  for (int o = 0; o < R; ++o) {
+  for (int o = 0; o < cpow(R, NS); ++o) {
+    float s0 = aO0[0][o / cpow(R, NS - 1)], s1 = aO1[0][o / cpow(R, NS - 1)];
+#pragma unroll
+    for (int t = 1; t < NS; ++t) {
+      const int dg = o / cpow(R, NS - 1 - t) % R;
+      s0 *= aO0[t][dg];
+      s1 *= aO1[t][dg];
+    }
 #pragma unroll
     for (int jL = 0; jL < R; ++jL) {
-      const float K0 = aO0[o] * aL0[jL], K1 = aO1[o] * aL1[jL];
+      const float K0 = s0 * aL0[jL], K1 = s1 * aL1[jL];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const float g = Gt[(o * R + jL) * R + r];

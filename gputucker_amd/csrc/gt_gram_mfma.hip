@@ -234,6 +234,61 @@ template <int R> struct ChunkGram {
 };
 
 // ------------------------------------------------------------ FP32 VALU delta
+// A chunk with NS outer slots and R^N <= 4096: two tiles per delta pass
+// (contract_pair), the Gram of each tile added in turn.
+template <int R, int NS>
+__device__ inline void valu_pairs(const GramArgs &a, ChunkGram<R> &gram, uint64_t cb, uint64_t ce,
+                                  int lane, float (*tiles)[16 * kDP],
+                                  uint32_t (*rows)[kWave]) {
+  constexpr int T = ChunkGram<R>::T;
+  uint32_t p_row[2], p_iL[2], p_io[2][kSlots];
+  float p_x[2];
+  auto load_pair = [&](uint64_t tb) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const uint64_t e = tb + h * kWave + lane;
+      const uint64_t ee = e < ce ? e : cb;
+      p_row[h] = a.idx_row[ee];
+      p_iL[h] = a.idx_L[ee];
+#pragma unroll
+      for (int t = 0; t < NS; ++t) p_io[h][t] = a.idx_O[t][ee];
+      p_x[h] = e < ce ? a.st.val[ee] : 0.f;
+    }
+  };
+  load_pair(cb);
+  for (uint64_t tb = cb; tb < ce; tb += 2 * kWave) {
+    const uint32_t row[2] = {p_row[0], p_row[1]};
+    const uint32_t iL[2] = {p_iL[0], p_iL[1]};
+    uint32_t io[2][kSlots];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int t = 0; t < kSlots; ++t) io[h][t] = t < NS ? p_io[h][t] : 0u;
+    const float x[2] = {p_x[0], p_x[1]};
+    if (tb + 2 * kWave < ce) load_pair(tb + 2 * kWave);
+    float d[2][R];
+    contract_pair<R, NS>(a.A_L, a.A_O, iL, io, core_operand(a.Gt), d[0], d[1]);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const bool valid = tb + h * kWave + lane < ce;
+      rows[h][lane] = valid ? row[h] : 0xFFFFFFFFu;
+#pragma unroll
+      for (int r = 0; r < 4 * T; ++r)
+        tiles[h][r * kDP + lane] = (r < R) ? (valid ? d[h][r] : 0.f) : (r == R ? x[h] : 0.f);
+    }
+    wave_sync();
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      const uint64_t th = tb + h * kWave;
+      if (th >= ce) break;
+      const int cnt = (ce - th < (uint64_t)kWave) ? (int)(ce - th) : kWave;
+      gram.add_tile(tiles[h], rows[h], th, cnt);
+    }
+    wave_sync();
+  }
+  gram.finish();
+}
+
 template <int R>
 __global__ void __launch_bounds__(256) k_gram_valu(GramArgs a, int wpb) {
   static_assert(R <= 15, "MFMA Gram covers R <= 15");
@@ -251,54 +306,22 @@ __global__ void __launch_bounds__(256) k_gram_valu(GramArgs a, int wpb) {
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
   ChunkGram<R> gram(a, c, cb, ce, lane);
 
-  if constexpr (R <= 16) {
-    if (slots == 1) { // N = 3: two tiles per delta pass (contract_pair)
-      uint32_t p_row[2], p_iL[2], p_io[2];
-      float p_x[2];
-      auto load_pair = [&](uint64_t tb) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const uint64_t e = tb + h * kWave + lane;
-          const uint64_t ee = e < ce ? e : cb;
-          p_row[h] = a.idx_row[ee];
-          p_iL[h] = a.idx_L[ee];
-          p_io[h] = a.idx_O[0][ee];
-          p_x[h] = e < ce ? a.st.val[ee] : 0.f;
-        }
-      };
-      load_pair(cb);
-      for (uint64_t tb = cb; tb < ce; tb += 2 * kWave) {
-        const uint32_t row[2] = {p_row[0], p_row[1]};
-        const uint32_t iL[2] = {p_iL[0], p_iL[1]}, io[2] = {p_io[0], p_io[1]};
-        const float x[2] = {p_x[0], p_x[1]};
-        if (tb + 2 * kWave < ce) load_pair(tb + 2 * kWave);
-        float d[2][R];
-        // the core is read through the constant address space with a zero
-        // offset the compiler cannot see through: its loads stay scalar and
-        // inside the loop (hoisted, its 4R^3 bytes would sit in vector
-        // registers)
-        uint32_t goff = 0;
-        asm volatile("" : "+s"(goff));
-        contract_pair<R>(a.A_L, a.A_O[0], iL, io, (const gt_cfloat *)(a.Gt + goff), d[0], d[1]);
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const bool valid = tb + h * kWave + lane < ce;
-          s_row[w][h][lane] = valid ? row[h] : 0xFFFFFFFFu;
-#pragma unroll
-          for (int r = 0; r < 4 * T; ++r)
-            s_tile[w][h][r * kDP + lane] = (r < R) ? (valid ? d[h][r] : 0.f) : (r == R ? x[h] : 0.f);
-        }
-        wave_sync();
-#pragma unroll 1
-        for (int h = 0; h < 2; ++h) {
-          const uint64_t th = tb + h * kWave;
-          if (th >= ce) break;
-          const int cnt = (ce - th < (uint64_t)kWave) ? (int)(ce - th) : kWave;
-          gram.add_tile(s_tile[w][h], s_row[w][h], th, cnt);
-        }
-        wave_sync();
-      }
-      gram.finish();
+  // orders 3 to 5 with small enough R^N: the unrolled two-tile contraction
+  if constexpr (pair_unrolled<R, 1>()) {
+    if (slots == 1) {
+      valu_pairs<R, 1>(a, gram, cb, ce, lane, s_tile[w], s_row[w]);
+      return;
+    }
+  }
+  if constexpr (pair_unrolled<R, 2>()) {
+    if (slots == 2) {
+      valu_pairs<R, 2>(a, gram, cb, ce, lane, s_tile[w], s_row[w]);
+      return;
+    }
+  }
+  if constexpr (pair_unrolled<R, 3>()) {
+    if (slots == 3) {
+      valu_pairs<R, 3>(a, gram, cb, ce, lane, s_tile[w], s_row[w]);
       return;
     }
   }
