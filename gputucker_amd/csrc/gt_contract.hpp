@@ -182,11 +182,10 @@ constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
 
 // Whether contract_pair's fully unrolled form is used for NS outer slots:
 // R^N multiply-adds per nonzero (N = NS + 2), twice over for the pair, must
-// stay a loop body the instruction cache holds, and the two nonzeros'
-// operands must leave room for three waves per SIMD (order 3 above R = 10
-// takes 194-256 registers).
+// stay a loop body the instruction cache holds (order 3 up to R = 16, order
+// 4 up to R = 8, order 5 up to R = 5).
 template <int R, int NS> constexpr bool pair_unrolled() {
-  return cpow(R, NS + 2) <= 4096 && (NS > 1 || R <= 10);
+  return cpow(R, NS + 2) <= 4096;
 }
 
 // contract_slots for NS outer slots (R^N <= 4096, so fully unrolled out of
