@@ -568,6 +568,132 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
   gram.finish();
 }
 
+// ------------------------------------------------------------ BF16, order 4
+// The refined BF16 delta of an order-4 tensor whose core table does not fit
+// in LDS beside the tiles (R = 8: 64 outer indices, a 64 KB table).  The
+// core operand of each MFMA comes from the table in global memory (it stays
+// in L2), two outer indices ahead; s_m[o] = A_0[i_0][o_0] * A_1[i_1][o_1] is
+// formed at its use from the two outer rows staged in LDS.  The next tile's
+// gathers
+// (split A_L words and both outer rows) are issued after this tile's stage
+// loop, so the core loads never wait behind them.
+template <int R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_gram_bf16_o4(GramArgs a, int wpb) {
+  static_assert(bf16r_single_mfma(R) && R >= 2, "one refined MFMA per block and o");
+  constexpr int RS = row_stride(R);
+  constexpr int T = ChunkGram<R>::T;
+  constexpr int O = R * R;
+  __shared__ float s_tile[4][16 * kDP];
+  __shared__ uint32_t s_row[4][kWave];
+  extern __shared__ __align__(16) float dyn[];
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
+  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
+  if (c >= a.chunk_end) return;
+  float *lrows = dyn + (size_t)w * 2 * R * kWave; // [slot][j][nonzero]
+  float *tile = s_tile[w];
+  const uint64_t cb = c * a.chunk;
+  const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
+  const int q = lane & 15, g = lane >> 4;
+  const s8 *tg = reinterpret_cast<const s8 *>(static_cast<const uint16_t *>(a.Gb) +
+                                              gb_k16_words(O));
+  ChunkGram<R> gram(a, c, cb, ce, lane);
+
+  uint32_t n_row, n_iL, n_io0, n_io1;
+  float n_x;
+  auto load_idx = [&](uint64_t tb) {
+    const uint64_t e = tb + lane;
+    const uint64_t ee = e < ce ? e : cb;
+    n_row = a.idx_row[ee];
+    n_iL = a.idx_L[ee];
+    n_io0 = a.idx_O[0][ee];
+    n_io1 = a.idx_O[1][ee];
+    n_x = e < ce ? a.st.val[ee] : 0.f;
+  };
+  // lane (q, g) takes, for block mb, the split words of nonzero 16 mb + q;
+  // each lane gathers its own nonzero's two outer rows
+  u4 g32[4];
+  float r0[RS], r1[RS];
+  auto gather = [&](uint32_t iL, uint32_t io0, uint32_t io1) {
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const uint32_t r = (uint32_t)__shfl((int)iL, mb * 16 + q);
+      g32[mb] = *reinterpret_cast<const u4 *>(a.As_L + ((size_t)r * 4 + g) * 4);
+    }
+    load_row<R>(a.A_O[0], io0, r0);
+    load_row<R>(a.A_O[1], io1, r1);
+  };
+  load_idx(cb);
+  uint32_t c_row = n_row;
+  float c_x = n_x;
+  gather(n_iL, n_io0, n_io1);
+  load_idx(cb + kWave);
+
+  for (uint64_t tb = cb; tb < ce; tb += kWave) {
+    const bool valid = tb + lane < ce;
+    const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
+    const uint32_t row = c_row;
+    const float x = c_x;
+    c_row = n_row;
+    c_x = n_x;
+    s_row[w][lane] = valid ? row : 0xFFFFFFFFu;
+    // this tile's outer rows to LDS, before the next gather reuses them
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      lrows[j * kWave + lane] = r0[j];
+      lrows[(R + j) * kWave + lane] = r1[j];
+    }
+    wave_sync();
+    float dacc[4][4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
+    s8 bA = tg[lane], bB = tg[kWave + lane];
+#pragma unroll 1
+    for (int o0 = 0; o0 < R; ++o0) {
+      float s1[4];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) s1[mb] = lrows[o0 * kWave + mb * 16 + q];
+#pragma unroll 2
+      for (int o1 = 0; o1 < R; ++o1) {
+        const int o = o0 * R + o1;
+        const b8 b = __builtin_bit_cast(b8, bA);
+        bA = bB;
+        if (o + 2 < O) bB = tg[(o + 2) * kWave + lane];
+        // U transposed: lane (q, g) gets U[16mb + q][o][4g + i]
+        f4 u[4];
+        const f4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+          u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, g32[mb]), z, 0,
+                                                          0, 0);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+          const float sv = s1[mb] * lrows[(R + o1) * kWave + mb * 16 + q];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
+        }
+      }
+    }
+    // the next tile's gathers and the stream loads of the tile after it
+    gather(n_iL, n_io0, n_io1);
+    load_idx(tb + 2 * kWave);
+    // lane (q, g) holds delta[16mb + q][4g + i]: columns to LDS, then x
+    if (g < T) {
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) tile[(4 * g + i) * kDP + mb * 16 + q] = dacc[mb][i];
+    }
+    tile[R * kDP + lane] = x;
+    wave_sync();
+    gram.add_tile(tile, s_row[w], tb, cnt);
+    wave_sync();
+  }
+  gram.finish();
+}
+
 // ------------------------------------------------------------ split factor
 // Row i of the split copy holds, for each lane group g, the 16 bytes of BF16
 // words that group's K=32 slots take (bf16r_plan), so the gather of one
@@ -608,6 +734,18 @@ template <int R, int KIND> void launch_impl(const GramArgs &a, int O, hipStream_
         KIND == kBf16 ? gb_k16_words(O) * 2
                       : (gb_k32_words(O) * (bf16r_single_mfma(R) ? 1 : 2)) * 2;
     if (per_wave + table + kStaticLdsBf16 > kLdsLimit || a.As_L == nullptr) { // tables too large
+      if constexpr (KIND == kBf16R && bf16r_single_mfma(R) && R >= 2) {
+        // order 4: the core operand from L2, the outer rows in LDS
+        const size_t rows_lds = (size_t)2 * R * kWave * sizeof(float);
+        if (a.slots == 2 && O == R * R && a.As_L != nullptr &&
+            4 * rows_lds + kStaticLdsBf16 <= kLdsLimit) {
+          const int wpb = 4;
+          const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
+          hipLaunchKernelGGL((k_gram_bf16_o4<R>), dim3(grid), dim3(64 * wpb), rows_lds * wpb, st,
+                             a, wpb);
+          return;
+        }
+      }
       launch_valu<R>(a, st);
       return;
     }

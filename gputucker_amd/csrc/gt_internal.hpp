@@ -148,7 +148,7 @@ struct GramArgs {
   uint64_t chunk_begin;       // this launch covers chunks [chunk_begin, chunk_end); the
   uint64_t chunk_end;         // stream pointers must be valid over those positions
   uint64_t rows;              // rows of this mode (row_ptr has rows + 1 entries)
-  const void *Gb;             // BF16 core operand table (hi, then lo): [o][lane][4]
+  const void *Gb;             // BF16 core operand tables: K16 hi, K32 refined, K32 lo (uint16)
   // the same operands in slot order, so the kernel never indexes by mode:
   // L = last mode other than n, outer t = the other modes ascending
   int slots;                             // N - 2
