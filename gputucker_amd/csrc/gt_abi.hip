@@ -384,12 +384,33 @@ float from_bf16(uint16_t h) {
 std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R) {
   const int O = (int)ipow(R, N > 2 ? N - 2 : 0);
   const size_t n16 = gb_k16_words(O), n32 = gb_k32_words(O);
-  std::vector<uint16_t> tab(n16 + 2 * n32, 0);
+  const size_t npair = bf16r_pair_table(R) ? gb_k32_pair_words(O) : 0;
+  std::vector<uint16_t> tab(n16 + 2 * n32 + npair, 0);
   auto g_at = [&](int o, int jL, int r) -> float {
-    return (jL < R && r < R) ? gt[((size_t)o * R + jL) * R + r] : 0.f;
+    return (o < O && jL < R && r < R) ? gt[((size_t)o * R + jL) * R + r] : 0.f;
   };
   auto hi = [&](float g) { return to_bf16(g); };
   auto lo = [&](float g) { return to_bf16(g - from_bf16(to_bf16(g))); };
+  // the K32 refined words of lane group g for output r of outer index o
+  auto k32_words = [&](uint16_t *k32, int o, int g, int r) {
+    switch (bf16r_plan(R, g)) {
+    case kPlanHL:
+      for (int t = 0; t < 8; ++t) k32[t] = hi(g_at(o, 4 * g + (t & 3), r));
+      break;
+    case kPlanHH:
+      for (int t = 0; t < 8; ++t) k32[t] = lo(g_at(o, t, r));
+      break;
+    case kPlanR10:
+      for (int t = 0; t < 4; ++t) k32[t] = hi(g_at(o, 8 + t, r));
+      k32[4] = hi(g_at(o, 8, r));
+      k32[5] = hi(g_at(o, 9, r));
+      k32[6] = lo(g_at(o, 8, r));
+      k32[7] = lo(g_at(o, 9, r));
+      break;
+    default:
+      break;
+    }
+  };
   for (int o = 0; o < O; ++o)
     for (int l = 0; l < kWave; ++l) {
       const int g = l / 16, r = l % 16;
@@ -398,24 +419,14 @@ std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R
         tab[at] = hi(g_at(o, 4 * g + t, r));                 // K16 hi
         tab[n16 + n32 + ((size_t)o * kWave + l) * 8 + t] = lo(g_at(o, 4 * g + t, r)); // K32 lo
       }
-      uint16_t *k32 = tab.data() + n16 + ((size_t)o * kWave + l) * 8;
-      switch (bf16r_plan(R, g)) {
-      case kPlanHL:
-        for (int t = 0; t < 8; ++t) k32[t] = hi(g_at(o, 4 * g + (t & 3), r));
-        break;
-      case kPlanHH:
-        for (int t = 0; t < 8; ++t) k32[t] = lo(g_at(o, t, r));
-        break;
-      case kPlanR10:
-        for (int t = 0; t < 4; ++t) k32[t] = hi(g_at(o, 8 + t, r));
-        k32[4] = hi(g_at(o, 8, r));
-        k32[5] = hi(g_at(o, 9, r));
-        k32[6] = lo(g_at(o, 8, r));
-        k32[7] = lo(g_at(o, 9, r));
-        break;
-      default:
-        break;
-      }
+      k32_words(tab.data() + n16 + ((size_t)o * kWave + l) * 8, o, g, r);
+    }
+  // K32 pair table (R <= 8): entry (p, lane) holds output row l % 16 =
+  // 8 * (o - 2p) + r of the outer pair p, so one MFMA covers two outer indices
+  for (size_t p = 0; npair && p < (size_t)(O + 1) / 2; ++p)
+    for (int l = 0; l < kWave; ++l) {
+      const int g = l / 16, q = l % 16;
+      k32_words(tab.data() + n16 + 2 * n32 + (p * kWave + l) * 8, (int)(2 * p) + q / 8, g, q % 8);
     }
   return tab;
 }

@@ -595,8 +595,11 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
   const uint64_t cb = c * a.chunk;
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
   const int q = lane & 15, g = lane >> 4;
-  const s8 *tg = reinterpret_cast<const s8 *>(static_cast<const uint16_t *>(a.Gb) +
-                                              gb_k16_words(O));
+  // R <= 8: the pair table, one MFMA per two outer indices; else the K32 table
+  constexpr bool kPair = bf16r_pair_table(R);
+  constexpr int NB = kPair ? (O + 1) / 2 : O; // MFMAs per 16 nonzeros
+  const s8 *tg = reinterpret_cast<const s8 *>(
+      static_cast<const uint16_t *>(a.Gb) + gb_k16_words(O) + (kPair ? 2 * gb_k32_words(O) : 0));
   ChunkGram<R> gram(a, c, cb, ce, lane);
 
   uint32_t n_row, n_iL, n_io0, n_io1;
@@ -650,41 +653,74 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
     s8 bA = tg[lane], bB = tg[kWave + lane];
-#pragma unroll 1
-    for (int o0 = 0; o0 < R; ++o0) {
-      float s1[4];
-#pragma unroll
-      for (int mb = 0; mb < 4; ++mb) s1[mb] = lrows[o0 * kWave + mb * 16 + q];
+    if constexpr (kPair) {
+      // MFMA p covers outer indices 2p (output rows 0..7) and 2p + 1 (rows
+      // 8..15): lane (q, g) gets U[16mb + q][o][r] with o = 2p + g / 2 and
+      // r = 4 (g % 2) + i, so it scales by s_m[o] of its own o
 #pragma unroll 2
-      for (int o1 = 0; o1 < R; ++o1) {
-        const int o = o0 * R + o1;
+      for (int p = 0; p < NB; ++p) {
         const b8 b = __builtin_bit_cast(b8, bA);
         bA = bB;
-        if (o + 2 < O) bB = tg[(o + 2) * kWave + lane];
-        // U transposed: lane (q, g) gets U[16mb + q][o][4g + i]
+        if (p + 2 < NB) bB = tg[(p + 2) * kWave + lane];
         f4 u[4];
         const f4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb)
           u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, g32[mb]), z, 0,
                                                           0, 0);
+        const int o = min(2 * p + (g >> 1), O - 1); // past O the core words are zero
+        const float *l0 = lrows + (o / R) * kWave + q, *l1 = lrows + (R + o % R) * kWave + q;
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
-          const float sv = s1[mb] * lrows[(R + o1) * kWave + mb * 16 + q];
+          const float sv = l0[mb * 16] * l1[mb * 16];
 #pragma unroll
           for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
+        }
+      }
+      // even and odd outer indices sit in lanes 32 apart: add the halves
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dacc[mb][i] += __shfl_xor(dacc[mb][i], 32);
+    } else {
+#pragma unroll 1
+      for (int o0 = 0; o0 < R; ++o0) {
+        float s1[4];
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) s1[mb] = lrows[o0 * kWave + mb * 16 + q];
+#pragma unroll 2
+        for (int o1 = 0; o1 < R; ++o1) {
+          const int o = o0 * R + o1;
+          const b8 b = __builtin_bit_cast(b8, bA);
+          bA = bB;
+          if (o + 2 < O) bB = tg[(o + 2) * kWave + lane];
+          // U transposed: lane (q, g) gets U[16mb + q][o][4g + i]
+          f4 u[4];
+          const f4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int mb = 0; mb < 4; ++mb)
+            u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, g32[mb]), z,
+                                                            0, 0, 0);
+#pragma unroll
+          for (int mb = 0; mb < 4; ++mb) {
+            const float sv = s1[mb] * lrows[(R + o1) * kWave + mb * 16 + q];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
+          }
         }
       }
     }
     // the next tile's gathers and the stream loads of the tile after it
     gather(n_iL, n_io0, n_io1);
     load_idx(tb + 2 * kWave);
-    // lane (q, g) holds delta[16mb + q][4g + i]: columns to LDS, then x
+    // lane (q, g) holds delta[16mb + q][4g + i] (with the pair table, for
+    // g < 2; columns 8 and up are zero): columns to LDS, then x
     if (g < T) {
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) tile[(4 * g + i) * kDP + mb * 16 + q] = dacc[mb][i];
+        for (int i = 0; i < 4; ++i)
+          tile[(4 * g + i) * kDP + mb * 16 + q] = (!kPair || g < 2) ? dacc[mb][i] : 0.f;
     }
     tile[R * kDP + lane] = x;
     wave_sync();

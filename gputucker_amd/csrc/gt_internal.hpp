@@ -191,9 +191,16 @@ __host__ __device__ constexpr int bf16r_plan(int R, int g) {
 }
 // BF16 core operand tables, in uint16 words: K16 hi table ([o][lane][4]),
 // then the K32 refined table ([o][lane][8]), then the K32 lo table
-// ([o][lane][8]; used when R > 10).
+// ([o][lane][8]; used when R > 10), then (R <= 8) the K32 pair table.
 __host__ __device__ constexpr size_t gb_k16_words(int O) { return (size_t)O * kWave * 4; }
 __host__ __device__ constexpr size_t gb_k32_words(int O) { return (size_t)O * kWave * 8; }
+// For R <= 8 a fourth table follows, the K32 pair table ([p][lane][8]): the
+// refined words of outer indices 2p and 2p + 1 in output rows 0..7 and 8..15,
+// so one MFMA per 16 nonzeros covers two outer indices.
+__host__ __device__ constexpr bool bf16r_pair_table(int R) { return R <= 8; }
+__host__ __device__ constexpr size_t gb_k32_pair_words(int O) {
+  return (size_t)((O + 1) / 2) * kWave * 8;
+}
 void launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
 // Split BF16 copy of a factor for the MFMA delta (R <= 15): 16 uint32 words
 // per row, the K=32 operand words of each lane group (bf16r_plan).
