@@ -568,29 +568,30 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
   gram.finish();
 }
 
-// ------------------------------------------------------------ BF16, order 4
-// The refined BF16 delta of an order-4 tensor whose core table does not fit
-// in LDS beside the tiles (R = 8: 64 outer indices, a 64 KB table).  The
+// ------------------------------------------------------------ BF16, orders 4-5
+// The refined BF16 delta of an order-4 or order-5 tensor (NS = 2 or 3 outer
+// slots) whose core table does not fit in LDS beside the tiles (order 4,
+// R = 8: 64 outer indices, a 64 KB table; order 5 needs R <= 8).  The
 // core operand of each MFMA comes from the table in global memory (it stays
-// in L2), two outer indices ahead; s_m[o] = A_0[i_0][o_0] * A_1[i_1][o_1] is
-// formed at its use from the two outer rows staged in LDS.  The next tile's
-// gathers
-// (split A_L words and both outer rows) are issued after this tile's stage
+// in L2), two outer indices ahead; s_m[o] = prod_t A_t[i_t][o_t] is formed at
+// its use from the outer rows staged in LDS.  The next tile's gathers
+// (split A_L words and the outer rows) are issued after this tile's stage
 // loop, so the core loads never wait behind them.
-template <int R>
+template <int R, int NS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_gram_bf16_o4(GramArgs a, int wpb) {
   static_assert(bf16r_single_mfma(R) && R >= 2, "one refined MFMA per block and o");
+  static_assert(NS == 2 || (NS == 3 && bf16r_pair_table(R)), "orders 4 and 5 (R <= 8)");
   constexpr int RS = row_stride(R);
   constexpr int T = ChunkGram<R>::T;
-  constexpr int O = R * R;
+  constexpr int O = cpow(R, NS);
   __shared__ float s_tile[4][16 * kDP];
   __shared__ uint32_t s_row[4][kWave];
   extern __shared__ __align__(16) float dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
   if (c >= a.chunk_end) return;
-  float *lrows = dyn + (size_t)w * 2 * R * kWave; // [slot][j][nonzero]
+  float *lrows = dyn + (size_t)w * NS * R * kWave; // [slot][j][nonzero]
   float *tile = s_tile[w];
   const uint64_t cb = c * a.chunk;
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
@@ -602,34 +603,34 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
       static_cast<const uint16_t *>(a.Gb) + gb_k16_words(O) + (kPair ? 2 * gb_k32_words(O) : 0));
   ChunkGram<R> gram(a, c, cb, ce, lane);
 
-  uint32_t n_row, n_iL, n_io0, n_io1;
+  uint32_t n_row, n_iL, n_io[NS];
   float n_x;
   auto load_idx = [&](uint64_t tb) {
     const uint64_t e = tb + lane;
     const uint64_t ee = e < ce ? e : cb;
     n_row = a.idx_row[ee];
     n_iL = a.idx_L[ee];
-    n_io0 = a.idx_O[0][ee];
-    n_io1 = a.idx_O[1][ee];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) n_io[t] = a.idx_O[t][ee];
     n_x = e < ce ? a.st.val[ee] : 0.f;
   };
   // lane (q, g) takes, for block mb, the split words of nonzero 16 mb + q;
-  // each lane gathers its own nonzero's two outer rows
+  // each lane gathers its own nonzero's outer rows
   u4 g32[4];
-  float r0[RS], r1[RS];
-  auto gather = [&](uint32_t iL, uint32_t io0, uint32_t io1) {
+  float ro[NS][RS];
+  auto gather = [&]() {
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-      const uint32_t r = (uint32_t)__shfl((int)iL, mb * 16 + q);
+      const uint32_t r = (uint32_t)__shfl((int)n_iL, mb * 16 + q);
       g32[mb] = *reinterpret_cast<const u4 *>(a.As_L + ((size_t)r * 4 + g) * 4);
     }
-    load_row<R>(a.A_O[0], io0, r0);
-    load_row<R>(a.A_O[1], io1, r1);
+#pragma unroll
+    for (int t = 0; t < NS; ++t) load_row<R>(a.A_O[t], n_io[t], ro[t]);
   };
   load_idx(cb);
   uint32_t c_row = n_row;
   float c_x = n_x;
-  gather(n_iL, n_io0, n_io1);
+  gather();
   load_idx(cb + kWave);
 
   for (uint64_t tb = cb; tb < ce; tb += kWave) {
@@ -642,10 +643,9 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
     s_row[w][lane] = valid ? row : 0xFFFFFFFFu;
     // this tile's outer rows to LDS, before the next gather reuses them
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-      lrows[j * kWave + lane] = r0[j];
-      lrows[(R + j) * kWave + lane] = r1[j];
-    }
+    for (int t = 0; t < NS; ++t)
+#pragma unroll
+      for (int j = 0; j < R; ++j) lrows[(t * R + j) * kWave + lane] = ro[t][j];
     wave_sync();
     float dacc[4][4];
 #pragma unroll
@@ -657,11 +657,10 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
       // MFMA p covers outer indices 2p (output rows 0..7) and 2p + 1 (rows
       // 8..15): lane (q, g) gets U[16mb + q][o][r] with o = 2p + g / 2 and
       // r = 4 (g % 2) + i, so it scales by s_m[o] of its own o
-#pragma unroll 2
-      for (int p = 0; p < NB; ++p) {
+      auto pair_step = [&](int p, int nb) {
         const b8 b = __builtin_bit_cast(b8, bA);
         bA = bB;
-        if (p + 2 < NB) bB = tg[(p + 2) * kWave + lane];
+        if (p + 2 < nb) bB = tg[(p + 2) * kWave + lane];
         f4 u[4];
         const f4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -669,20 +668,37 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
           u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, g32[mb]), z, 0,
                                                           0, 0);
         const int o = min(2 * p + (g >> 1), O - 1); // past O the core words are zero
-        const float *l0 = lrows + (o / R) * kWave + q, *l1 = lrows + (R + o % R) * kWave + q;
+        const float *l[NS];
+#pragma unroll
+        for (int t = 0; t < NS; ++t)
+          l[t] = lrows + (t * R + o / cpow(R, NS - 1 - t) % R) * kWave + q;
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
-          const float sv = l0[mb * 16] * l1[mb * 16];
+          float sv = l[0][mb * 16];
+#pragma unroll
+          for (int t = 1; t < NS; ++t) sv *= l[t][mb * 16];
 #pragma unroll
           for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
         }
+      };
+      if constexpr (NB % 2 == 0) {
+#pragma unroll 2
+        for (int p = 0; p < NB; ++p) pair_step(p, NB);
+      } else {
+        // an odd trip count the compiler cannot see, so it keeps the loop
+        // rolled (it would unroll an odd constant count fully, issue every
+        // core load at once and spill)
+        int nb = NB;
+        asm volatile("" : "+s"(nb));
+#pragma unroll 1
+        for (int p = 0; p < nb; ++p) pair_step(p, nb);
       }
       // even and odd outer indices sit in lanes 32 apart: add the halves
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
         for (int i = 0; i < 4; ++i) dacc[mb][i] += __shfl_xor(dacc[mb][i], 32);
-    } else {
+    } else if constexpr (NS == 2) {
 #pragma unroll 1
       for (int o0 = 0; o0 < R; ++o0) {
         float s1[4];
@@ -711,7 +727,7 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
       }
     }
     // the next tile's gathers and the stream loads of the tile after it
-    gather(n_iL, n_io0, n_io1);
+    gather();
     load_idx(tb + 2 * kWave);
     // lane (q, g) holds delta[16mb + q][4g + i] (with the pair table, for
     // g < 2; columns 8 and up are zero): columns to LDS, then x
@@ -771,15 +787,23 @@ template <int R, int KIND> void launch_impl(const GramArgs &a, int O, hipStream_
                       : (gb_k32_words(O) * (bf16r_single_mfma(R) ? 1 : 2)) * 2;
     if (per_wave + table + kStaticLdsBf16 > kLdsLimit || a.As_L == nullptr) { // tables too large
       if constexpr (KIND == kBf16R && bf16r_single_mfma(R) && R >= 2) {
-        // order 4: the core operand from L2, the outer rows in LDS
-        const size_t rows_lds = (size_t)2 * R * kWave * sizeof(float);
-        if (a.slots == 2 && O == R * R && a.As_L != nullptr &&
-            4 * rows_lds + kStaticLdsBf16 <= kLdsLimit) {
-          const int wpb = 4;
-          const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
-          hipLaunchKernelGGL((k_gram_bf16_o4<R>), dim3(grid), dim3(64 * wpb), rows_lds * wpb, st,
-                             a, wpb);
-          return;
+        // orders 4 and 5: the core operand from L2, the outer rows in LDS
+        const int wpb = 4;
+        const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
+        const size_t rows_lds = (size_t)a.slots * R * kWave * sizeof(float);
+        if (a.As_L != nullptr && wpb * rows_lds + kStaticLdsBf16 <= kLdsLimit) {
+          if (a.slots == 2 && O == R * R) {
+            hipLaunchKernelGGL((k_gram_bf16_o4<R, 2>), dim3(grid), dim3(64 * wpb),
+                               rows_lds * wpb, st, a, wpb);
+            return;
+          }
+          if constexpr (bf16r_pair_table(R)) {
+            if (a.slots == 3 && O == R * R * R) {
+              hipLaunchKernelGGL((k_gram_bf16_o4<R, 3>), dim3(grid), dim3(64 * wpb),
+                                 rows_lds * wpb, st, a, wpb);
+              return;
+            }
+          }
         }
       }
       launch_valu<R>(a, st);

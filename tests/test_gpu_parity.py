@@ -168,15 +168,17 @@ def test_error_paths():
 @pytest.mark.parametrize("prec,tol_gram", [("bf16", 3e-2), ("bf16r", 2e-4), ("fp32", 1e-5)])
 @pytest.mark.parametrize("N,R,part", [(3, 10, [2, 1, 2]), (4, 5, [1, 2, 1, 1]), (2, 8, [1, 3]),
                                       (3, 13, [2, 1, 2]), (3, 15, [1, 1, 1]), (3, 7, [1, 2, 1]),
-                                      (2, 13, [2, 1]), (4, 7, [1, 2, 1, 1]), (4, 9, [1, 1, 2, 1])])
+                                      (2, 13, [2, 1]), (4, 7, [1, 2, 1, 1]), (4, 9, [1, 1, 2, 1]),
+                                      (5, 4, [1, 1, 1, 2, 1])])
 def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
     """BF16 MFMA delta (plain and with the split-BF16 refinement; above
     R = 10 the refinement takes a second MFMA per tile; at order 4 with
-    R = 7 and 9 the refined core table exceeds LDS and the order-4 kernel
-    reads it from L2, packing two outer indices per MFMA at R = 7) and the
+    R = 7 and 9, and at order 5 with R = 4, the refined core table exceeds
+    LDS and the core operand comes from L2, two outer indices per MFMA up to
+    R = 8) and the
     FP32 VALU delta (two tiles per pass) feeding the MFMA Gram, against the
     FP64 oracle's B and C."""
-    dims = [90, 70, 50, 30][:N]
+    dims = [90, 70, 50, 30, 20][:N]
     e, t, A, G = _setup(N * 13 + R, dims, 12000, R, part, skew=1.2, precision=prec,
                         chunk_tiles=1)
     lam = gt.default_lambda()
