@@ -10,7 +10,7 @@ device synchronisations; t_mode is the median over the timed ALS iterations
 of (iteration time without the fit) / N, and value = nnz / t_mode.  The fit
 is computed after the timed region.
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c4] [--precision fp32]
+    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5s] [--precision fp32]
 """
 from __future__ import annotations
 
@@ -41,6 +41,11 @@ CONFIGS = {
                dims=[200_000] * 3 + [1_000], nnz=200_000_000,
                modes=[("zipf", 1.2)] * 3 + ["uniform"], values="lognormal", rank=8, part=None,
                iters=3, lam=None),
+    # order 5, R = 5 at a size the 64-bit coordinate keys of the generator
+    # reach (the 5e4^4 x 200 index space needs 71 bits), in core
+    "c5s": dict(name="5-order 1e4^4 x 200, 2e8 nnz uniform, R=5", dims=[10_000] * 4 + [200],
+                nnz=200_000_000, modes=None, values="uniform", rank=5, part=None, iters=3,
+                lam=None),
     # R = 32 (the FP32 delta and the lane-per-entry Gram) with power-law rows
     # in mode 0; Zipf(1.5) in every mode would need about 2e10 draws for 3e8
     # distinct coordinates
