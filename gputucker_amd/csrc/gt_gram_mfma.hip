@@ -598,7 +598,7 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
   const int q = lane & 15, g = lane >> 4;
   // R <= 8: the pair table, one MFMA per two outer indices; else the K32 table
   constexpr bool kPair = bf16r_pair_table(R);
-  constexpr int NB = kPair ? (O + 1) / 2 : O; // MFMAs per 16 nonzeros
+  constexpr int NB = kPair ? gb_pairs(O) : O; // MFMAs per 16 nonzeros
   const s8 *tg = reinterpret_cast<const s8 *>(
       static_cast<const uint16_t *>(a.Gb) + gb_k16_words(O) + (kPair ? 2 * gb_k32_words(O) : 0));
   ChunkGram<R> gram(a, c, cb, ce, lane);
@@ -681,18 +681,9 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
           for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
         }
       };
-      if constexpr (NB % 2 == 0) {
+      static_assert(NB % 2 == 0, "the pair table has an even number of pairs");
 #pragma unroll 2
-        for (int p = 0; p < NB; ++p) pair_step(p, NB);
-      } else {
-        // an odd trip count the compiler cannot see, so it keeps the loop
-        // rolled (it would unroll an odd constant count fully, issue every
-        // core load at once and spill)
-        int nb = NB;
-        asm volatile("" : "+s"(nb));
-#pragma unroll 1
-        for (int p = 0; p < nb; ++p) pair_step(p, nb);
-      }
+      for (int p = 0; p < NB; ++p) pair_step(p, NB);
       // even and odd outer indices sit in lanes 32 apart: add the halves
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)

@@ -198,8 +198,11 @@ __host__ __device__ constexpr size_t gb_k32_words(int O) { return (size_t)O * kW
 // refined words of outer indices 2p and 2p + 1 in output rows 0..7 and 8..15,
 // so one MFMA per 16 nonzeros covers two outer indices.
 __host__ __device__ constexpr bool bf16r_pair_table(int R) { return R <= 8; }
+// pairs padded to an even count (zero words), so the kernel's loop over them
+// unrolls by two
+__host__ __device__ constexpr int gb_pairs(int O) { return ((O + 1) / 2 + 1) / 2 * 2; }
 __host__ __device__ constexpr size_t gb_k32_pair_words(int O) {
-  return (size_t)((O + 1) / 2) * kWave * 8;
+  return (size_t)gb_pairs(O) * kWave * 8;
 }
 void launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
 // Split BF16 copy of a factor for the MFMA delta (R <= 15): 16 uint32 words
