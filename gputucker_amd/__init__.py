@@ -27,8 +27,10 @@ STATUS = {0: "GT_OK", 1: "GT_ERR_ARG", 2: "GT_ERR_HIP", 3: "GT_ERR_STATE",
 EXPORTS = [
     "gt_default_opts", "gt_plan_partition", "gt_schedule", "gt_create", "gt_destroy",
     "gt_last_error", "gt_set_coo", "gt_partition", "gt_get_partition_info", "gt_get_block_nnz", "gt_get_block",
-    "gt_partition_dump", "gt_set_init", "gt_mode_update", "gt_fit", "gt_iteration", "gt_run",
-    "gt_get_factor", "gt_set_factor", "gt_get_delta", "gt_get_gram", "gt_get_recon",
+    "gt_partition_dump", "gt_set_init", "gt_mode_update", "gt_fit", "gt_iteration",
+    "gt_run",
+    "gt_get_factor", "gt_set_factor", "gt_get_delta", "gt_get_stream_delta", "gt_get_gram",
+    "gt_get_recon",
     "gt_get_timings", "gt_synchronize", "gt_version",
 ]
 
@@ -38,7 +40,7 @@ class GtOpts(C.Structure):
         ("device", C.c_int), ("order", C.c_int), ("rank", C.c_int), ("lambda_", C.c_double),
         ("max_iters", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
         ("partition", C.c_uint32 * MAX_ORDER), ("device_budget", C.c_uint64),
-        ("chunk_tiles", C.c_int), ("profile", C.c_int),
+        ("chunk_tiles", C.c_int), ("profile", C.c_int), ("ref_fit", C.c_int),
     ]
 
 
@@ -106,6 +108,7 @@ def lib():
         L.gt_set_factor.argtypes = [vp, C.c_int, vp]
         L.gt_get_delta.argtypes = [vp, C.c_int, vp]
         L.gt_get_gram.argtypes = [vp, C.c_int, vp, vp]
+        L.gt_get_stream_delta.argtypes = [vp, C.c_int, vp, C.POINTER(vp)]
         L.gt_get_recon.argtypes = [vp, vp]
         L.gt_get_timings.argtypes = [vp, C.POINTER(GtTimings)]
         L.gt_synchronize.argtypes = [vp]
@@ -158,7 +161,7 @@ class Engine:
     """One device context.  Mirrors the reference's TuckerDecomposition flow."""
 
     def __init__(self, order, rank, device=0, lam=None, max_iters=3, tol=None, precision="fp32",
-                 partition=None, device_budget=0, chunk_tiles=0, profile=False):
+                 partition=None, device_budget=0, chunk_tiles=0, profile=False, ref_fit=False):
         L = lib()
         o = GtOpts()
         L.gt_default_opts(C.byref(o))
@@ -175,6 +178,7 @@ class Engine:
         o.device_budget = int(device_budget)
         o.chunk_tiles = int(chunk_tiles)
         o.profile = 1 if profile else 0
+        o.ref_fit = 1 if ref_fit else 0
         self.opts = o
         self.order, self.rank = int(order), int(rank)
         self._h = C.c_void_p()
@@ -290,6 +294,15 @@ class Engine:
         out = np.zeros((max(self.nnz, 1), self.rank), np.float32)
         self._check(lib().gt_get_delta(self._h, int(mode), out.ctypes.data))
         return out[: self.nnz]
+
+    def get_stream_delta(self, mode):
+        """The mode update kernel's own deltas of mode `mode` (stream order)
+        and the stream's global indices (N, nnz)."""
+        out = np.zeros((max(self.nnz, 1), self.rank), np.float32)
+        idx = np.zeros((self.order, max(self.nnz, 1)), np.uint32)
+        ptrs = (C.c_void_p * self.order)(*[idx[k].ctypes.data for k in range(self.order)])
+        self._check(lib().gt_get_stream_delta(self._h, int(mode), out.ctypes.data, ptrs))
+        return out[: self.nnz], idx[:, : self.nnz]
 
     def get_recon(self):
         out = np.zeros(max(self.nnz, 1), np.float32)

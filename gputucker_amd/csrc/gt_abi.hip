@@ -176,6 +176,7 @@ struct gt_ctx {
   uint64_t B = 0;
   std::vector<uint64_t> block_nnz, block_off;
   double norm = 0.0;
+  float norm_ref = 0.f; // opts.ref_fit: serial float norm in input order
   double ms_partition = 0.0;
   DevBuf<uint64_t> d_block_off;
   DevBuf<uint32_t> coo_idx[GT_MAX_ORDER];
@@ -197,6 +198,8 @@ struct gt_ctx {
   DevBuf<double> gram, bpart0, bpart1, err_part, scalar;
   DevBuf<uint32_t> chunk_tail; // per chunk: row continuing into the next chunk, or ~0u
   DevBuf<double> fit_part;     // residual partials of the row solves
+  DevBuf<float> xhat;          // opts.ref_fit: the reconstruction of every nonzero, block order
+  float *dbg_delta = nullptr;  // test hook: the Gram kernels' deltas go here (stream order)
   uint64_t chunk = 0, nchunks = 0;
   // out-of-core pipeline: the per-mode streams live in pinned host memory and
   // are pumped through two device staging buffers on a copy stream
@@ -578,6 +581,7 @@ GramArgs gram_args(gt_ctx *c, int n) {
   a.idx_L = c->st_idx[n][L].p;
   a.A_L = c->A[L].p;
   a.As_L = c->As[L].p; // null unless a BF16 precision is selected
+  a.delta_out = c->dbg_delta;
   int t = 0;
   for (int k = 0; k < c->N; ++k) {
     if (k == n || k == L) continue;
@@ -743,18 +747,28 @@ CooArgs coo_args(gt_ctx *c, int mode) {
   return a;
 }
 
-double compute_fit(gt_ctx *c) {
+// FP64 partials the reconstruction pass of compute_fit writes: one grid's
+// worth in core, one per staged segment out of core.
+uint64_t fit_partials_needed(const gt_ctx *c) {
+  if (!c->streaming) return kErrPartialsPerSegment;
+  return ((c->nnz + c->seg_nnz - 1) / c->seg_nnz) * kErrPartialsPerSegment;
+}
+
+// The reconstruction pass (reconstruction.cuh:58-81, 259-276): the error
+// partials go to `part` (fit_partials_needed(c) doubles) and their sum to
+// `scal` (one double), both device memory.
+double compute_fit(gt_ctx *c, double *part, double *scal) {
   require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
   GT_HIP(hipEventRecord(c->ev[3], c->s));
   if (!c->streaming) {
     CooArgs a = coo_args(c, c->N - 1);
-    a.err_partials = c->err_part.p;
+    a.err_partials = part;
     int g = launch_coo_contract(c->R, a, c->s);
-    reduce_f64(c->err_part.p, g, c->scalar.p, c->s);
+    reduce_f64(part, g, scal, c->s);
   } else { // any order of the nonzeros gives the same sum: stream mode N-1's copy
     const int n = c->N - 1, L = c->N - 2;
     const uint64_t nseg = (c->nnz + c->seg_nnz - 1) / c->seg_nnz;
-    fill_f64(c->err_part.p, 0.0, nseg * kErrPartialsPerSegment, c->s);
+    fill_f64(part, 0.0, nseg * kErrPartialsPerSegment, c->s);
     for (uint64_t k = 0; k < nseg; ++k) {
       const int b = (int)(k & 1);
       const auto [lo, hi] = stage_segment(c, n, k);
@@ -771,14 +785,14 @@ double compute_fit(gt_ctx *c) {
       }
       a.val = c->stage_val[b].p;
       a.Gt = c->Gt[n].p;
-      a.err_partials = c->err_part.p + k * kErrPartialsPerSegment;
+      a.err_partials = part + k * kErrPartialsPerSegment;
       launch_coo_contract(c->R, a, c->s);
       GT_HIP(hipEventRecord(c->ev_comp[b], c->s));
     }
-    reduce_f64(c->err_part.p, (int)(nseg * kErrPartialsPerSegment), c->scalar.p, c->s);
+    reduce_f64(part, (int)(nseg * kErrPartialsPerSegment), scal, c->s);
   }
   double err = 0.0;
-  GT_HIP(hipMemcpyAsync(&err, c->scalar.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
+  GT_HIP(hipMemcpyAsync(&err, scal, sizeof(double), hipMemcpyDeviceToHost, c->s));
   GT_HIP(hipEventRecord(c->ev[4], c->s));
   GT_HIP(hipStreamSynchronize(c->s));
   float ms = 0.f;
@@ -788,11 +802,51 @@ double compute_fit(gt_ctx *c) {
   return 1.0 - std::sqrt(err) / c->norm;
 }
 
+// The fit in the reference's value_t arithmetic (reconstruction.cuh:259-276):
+// xhat of every nonzero in block order from the reconstruction pass, then on
+// the host a serial float sum of (x - xhat)^2 over blocks and nonzeros in
+// order, and fit = 1.0f - sqrtf(err) / norm in float, the norm being the
+// serial float norm of gt_set_coo (tensor_manager.tpp:263-293).
+double compute_fit_ref(gt_ctx *c) {
+  require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
+  require(!c->streaming, GT_ERR_STATE,
+          "the reference-arithmetic fit needs the block COO resident (in-core layout)");
+  GT_HIP(hipEventRecord(c->ev[3], c->s));
+  c->xhat.alloc(std::max<uint64_t>(c->nnz, 1));
+  CooArgs a = coo_args(c, c->N - 1);
+  a.xhat_out = c->xhat.p;
+  launch_coo_contract(c->R, a, c->s);
+  std::vector<float> xh(c->nnz), x(c->nnz);
+  if (c->nnz) {
+    GT_HIP(hipMemcpyAsync(xh.data(), c->xhat.p, c->nnz * sizeof(float), hipMemcpyDeviceToHost,
+                          c->s));
+    GT_HIP(hipMemcpyAsync(x.data(), c->coo_val.p, c->nnz * sizeof(float), hipMemcpyDeviceToHost,
+                          c->s));
+  }
+  GT_HIP(hipStreamSynchronize(c->s));
+  float err = 0.0f;
+  for (uint64_t e = 0; e < c->nnz; ++e) {
+    const float d = x[e] - xh[e];
+    err += d * d;
+  }
+  GT_HIP(hipEventRecord(c->ev[4], c->s));
+  GT_HIP(hipStreamSynchronize(c->s));
+  float ms = 0.f;
+  GT_HIP(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+  c->tm.ms_fit = ms;
+  if (c->norm_ref == 0.f) return 1.0;
+  return (double)(1.0f - std::sqrt(err) / c->norm_ref);
+}
+
 // One ALS iteration (update.cuh:146-198 then reconstruction.cuh:228-277).
 // The fit comes from the last mode's row solves: after that update,
 // sum_e (x_e - xhat_e)^2 = |x|^2 + sum_i (a_i^T G_i a_i - 2 a_i^T C_i), so no
 // reconstruction pass over the nonzeros is needed.
 double iteration(gt_ctx *c) {
+  if (c->o.ref_fit) {
+    for (int n = 0; n < c->N; ++n) mode_update(c, n, false, false);
+    return compute_fit_ref(c);
+  }
   // the N updates are queued back to back; the host waits once, for the fit
   for (int n = 0; n + 1 < c->N; ++n) mode_update(c, n, false, false);
   const uint64_t parts = mode_update(c, c->N - 1, true, false);
@@ -1131,6 +1185,11 @@ int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const floa
         GT_HIP(hipMemcpyAsync(ctx->in_idx[k].p, idx[k], nnz * sizeof(uint32_t),
                               hipMemcpyHostToDevice, ctx->s));
     }
+    if (ctx->o.ref_fit) { // tensor_manager.tpp:263,289,293: serial, in value_t
+      float ss = 0.0f;
+      for (uint64_t e = 0; e < nnz; ++e) ss += vals[e] * vals[e];
+      ctx->norm_ref = std::sqrt(ss);
+    }
     ctx->in_val.alloc(nnz);
     if (nnz)
       GT_HIP(hipMemcpyAsync(ctx->in_val.p, vals, nnz * sizeof(float), hipMemcpyHostToDevice,
@@ -1284,7 +1343,7 @@ int gt_mode_update(gt_ctx *ctx, int mode) {
 
 int gt_fit(gt_ctx *ctx, double *fit) {
   return guarded(ctx, [&] {
-    double f = compute_fit(ctx);
+    double f = ctx->o.ref_fit ? compute_fit_ref(ctx) : compute_fit(ctx, ctx->err_part.p, ctx->scalar.p);
     if (fit) *fit = f;
   });
 }
@@ -1346,6 +1405,31 @@ int gt_get_delta(gt_ctx *ctx, int mode, float *out) {
     launch_coo_contract(ctx->R, a, ctx->s);
     GT_HIP(hipMemcpyAsync(out, d.p, ctx->nnz * ctx->R * sizeof(float), hipMemcpyDeviceToHost,
                           ctx->s));
+    GT_HIP(hipStreamSynchronize(ctx->s));
+  });
+}
+
+int gt_get_stream_delta(gt_ctx *ctx, int mode, float *delta, uint32_t *const *idx) {
+  return guarded(ctx, [&] {
+    require(ctx->partitioned && ctx->initialized, GT_ERR_STATE, "partition and init first");
+    require(!ctx->streaming, GT_ERR_STATE, "the streams are not resident (out-of-core mode)");
+    require(mode >= 0 && mode < ctx->N && delta, GT_ERR_ARG, "bad argument");
+    DevBuf<float> d;
+    d.alloc(std::max<uint64_t>(ctx->nnz * ctx->R, 1));
+    ctx->dbg_delta = d.p;
+    try {
+      run_gram(ctx, mode);
+    } catch (...) {
+      ctx->dbg_delta = nullptr;
+      throw;
+    }
+    ctx->dbg_delta = nullptr;
+    GT_HIP(hipMemcpyAsync(delta, d.p, ctx->nnz * ctx->R * sizeof(float), hipMemcpyDeviceToHost,
+                          ctx->s));
+    for (int k = 0; idx && k < ctx->N; ++k)
+      if (idx[k])
+        GT_HIP(hipMemcpyAsync(idx[k], ctx->st_idx[mode][k].p, ctx->nnz * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, ctx->s));
     GT_HIP(hipStreamSynchronize(ctx->s));
   });
 }

@@ -106,6 +106,10 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
     s_row[w][lane] = rows[ee];
     wave_sync();
     const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
+    if (a.delta_out != nullptr && lane < cnt) { // test hook, uniform branch
+#pragma unroll
+      for (int r = 0; r < R; ++r) a.delta_out[(tb + lane) * R + r] = tile[lane * RP + r];
+    }
     for (int j = 0; j < cnt; ++j) {
       const uint32_t row = __builtin_amdgcn_readfirstlane(s_row[w][j]);
       if (row != cur) {

@@ -233,6 +233,18 @@ template <int R> struct ChunkGram {
   }
 };
 
+// Test hook: the deltas of tile [tb, tb + cnt) (columns < R of `tile`) to
+// a.delta_out in stream order.  a.delta_out is a kernel argument, so the
+// branch is uniform; it is null on the ALS path.
+template <int R>
+__device__ inline void dump_tile(const GramArgs &a, const float *tile, uint64_t tb, int cnt,
+                                 int lane) {
+  if (a.delta_out != nullptr && lane < cnt) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) a.delta_out[(tb + lane) * R + r] = tile[r * kDP + lane];
+  }
+}
+
 // ------------------------------------------------------------ FP32 VALU delta
 // A chunk with NS outer slots and R^N <= 4096: two tiles per delta pass
 // (contract_pair), the Gram of each tile added in turn.
@@ -282,6 +294,7 @@ __device__ inline void valu_pairs(const GramArgs &a, ChunkGram<R> &gram, uint64_
       const uint64_t th = tb + h * kWave;
       if (th >= ce) break;
       const int cnt = (ce - th < (uint64_t)kWave) ? (int)(ce - th) : kWave;
+      dump_tile<R>(a, tiles[h], th, cnt, lane);
       gram.add_tile(tiles[h], rows[h], th, cnt);
     }
     wave_sync();
@@ -356,6 +369,7 @@ __global__ void __launch_bounds__(256) k_gram_valu(GramArgs a, int wpb) {
     for (int r = 0; r < 4 * T; ++r)
       tile[r * kDP + lane] = (r < R) ? (valid ? d[r] : 0.f) : (r == R ? x : 0.f);
     wave_sync();
+    dump_tile<R>(a, tile, tb, cnt, lane);
     gram.add_tile(tile, s_row[w][0], tb, cnt);
     wave_sync();
   }
@@ -562,6 +576,7 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
     }
     tile[R * kDP + lane] = x;
     wave_sync();
+    dump_tile<R>(a, tile, tb, cnt, lane);
     gram.add_tile(tile, s_row[w], tb, cnt);
     wave_sync();
   }
@@ -731,6 +746,7 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
     }
     tile[R * kDP + lane] = x;
     wave_sync();
+    dump_tile<R>(a, tile, tb, cnt, lane);
     gram.add_tile(tile, s_row[w], tb, cnt);
     wave_sync();
   }

@@ -158,6 +158,7 @@ struct GramArgs {
   const float *A_L;
   const float *A_O[kMaxOrder - 2];
   const uint32_t *As_L;                  // split BF16 copy of A_L (16 words per row) or null
+  float *delta_out;                      // test hook: null, or nnz x R deltas in stream order
 };
 void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s);
 void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);

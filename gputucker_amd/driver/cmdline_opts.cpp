@@ -37,6 +37,8 @@ void CommandLineOptions::PrintHelp() const {
                "  --output prefix               Write factors, core and fit history as text files\n"
                "  --resume prefix               Start from the factors and core of an --output run\n"
                "  --fused-fit                   Take each iteration's fit from the row solves\n"
+               "  --ref-fit                     Fit in the reference's float arithmetic (serial sums)\n"
+               "  --dump-init prefix            Write the initial factors and core, then exit (no GPU)\n"
                "  --cache                       Read/write a binary copy of the input (<input>.gtcoo)\n"
                "  --parse-only                  Parse the input, print its summary, exit (no GPU)\n";
 }
@@ -153,6 +155,11 @@ CommandLineOptions::ReturnStatus CommandLineOptions::Parse(int argc, char *argv[
       _resume = v;
     } else if (a == "--fused-fit") {
       _fused_fit = true;
+    } else if (a == "--ref-fit") {
+      _ref_fit = true;
+    } else if (a == "--dump-init") {
+      if (!need(v)) return OPTS_FAILURE;
+      _dump_init = v;
     } else if (a == "--cache") {
       _cache = true;
     } else {

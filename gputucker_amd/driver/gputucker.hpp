@@ -52,6 +52,8 @@ public:
   bool get_cache() const { return _cache; }
   const std::string &get_resume_prefix() const { return _resume; }
   bool get_fused_fit() const { return _fused_fit; }
+  bool get_ref_fit() const { return _ref_fit; }
+  const std::string &get_dump_init_prefix() const { return _dump_init; }
   void PrintHelp() const;
 
 private:
@@ -72,6 +74,8 @@ private:
   bool _cache = false;       // read/write a binary COO copy beside the input
   std::string _resume;       // start from factors and core written by --output
   bool _fused_fit = false;   // fit from the row solves (gt_iteration)
+  bool _ref_fit = false;     // fit in the reference's float arithmetic (gt_opts.ref_fit)
+  std::string _dump_init;    // write the initial factors and core, then stop (no GPU)
 };
 
 template <typename IndexType, typename ValueType> class Block {

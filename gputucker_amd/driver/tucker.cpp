@@ -78,6 +78,14 @@ void TuckerDecomposition(tensor_t *t, const CommandLineOptions &o) {
     printf("\t... Read the core tensor from %s_core.txt\n", resume.c_str());
     read_matrix(resume + "_core.txt", core.data(), core_n / R, R);
   }
+  if (!o.get_dump_init_prefix().empty()) { // the draws above, for checking; no GPU
+    for (int a = 0; a < N; ++a)
+      write_matrix(o.get_dump_init_prefix() + "_factor" + std::to_string(a) + ".txt", A[a].data(),
+                   t->dims[a], R);
+    write_matrix(o.get_dump_init_prefix() + "_core.txt", core.data(), core_n / R, R);
+    printf("DONE\n");
+    return;
+  }
   gt_opts opts;
   gt_default_opts(&opts);
   opts.device = o.get_device();
@@ -88,6 +96,7 @@ void TuckerDecomposition(tensor_t *t, const CommandLineOptions &o) {
   opts.tol = o.get_tol();
   opts.precision = o.get_precision();
   opts.device_budget = o.get_device_budget();
+  opts.ref_fit = o.get_ref_fit() ? 1 : 0;
   for (size_t a = 0; a < o.get_partition().size(); ++a) opts.partition[a] = o.get_partition()[a];
   printf("\t... Initialize the GPU engine\n");
   if (gt_create(&t->engine, &opts) != GT_OK)

@@ -63,6 +63,14 @@ typedef struct gt_opts {
   int chunk_tiles;                  /* 64-nonzero tiles per Gram chunk; 0 = from the nonzero
                                        count: about 16384 chunks, 1 to 16 tiles each */
   int profile;                      /* 1: time the Gram kernel with HIP events */
+  int ref_fit;                      /* 0: the fit from FP64 sums.  1: the fit in the reference's
+                                       value_t arithmetic: the norm as a serial float sum of x^2
+                                       in input order (tensor_manager.tpp:263-293), the error as
+                                       a serial float sum of (x - xhat)^2 in block order, and
+                                       fit = 1.0f - sqrtf(err) / norm in float
+                                       (reconstruction.cuh:259-276).  Needs the block COO
+                                       resident (in core); gt_iteration then runs the
+                                       reconstruction pass instead of the fused fit */
 } gt_opts;
 
 typedef struct gt_partition_info {
@@ -176,6 +184,11 @@ int gt_get_delta(gt_ctx *ctx, int mode, float *out);
 /* Test hook: B (rows x R x R, lambda on the diagonal) and C (rows x R) of mode n
  * over the padded row space P_n * block_dims[n], as ComputingBC leaves them. */
 int gt_get_gram(gt_ctx *ctx, int mode, double *B, double *C);
+/* Test hook: the deltas of mode n as the mode update's own delta + Gram
+ * kernel forms them (the precision selected), one row of R per nonzero in the
+ * order of mode n's row-sorted stream; idx (may be NULL, or N arrays of nnz)
+ * receives the stream's global indices, which name each nonzero. */
+int gt_get_stream_delta(gt_ctx *ctx, int mode, float *delta, uint32_t *const *idx);
 /* Test hook: xhat for every nonzero in block order. */
 int gt_get_recon(gt_ctx *ctx, float *out);
 

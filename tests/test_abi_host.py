@@ -53,6 +53,18 @@ def test_planner_matches_oracle(case):
     assert (a["streams"], a["avail_nnz"], a["type"]) == (b["streams"], b["avail_nnz"], b["type"])
 
 
+@pytest.mark.parametrize("case", __import__("tests.test_oracle_partition",
+                                            fromlist=["PLAN_CASES"]).PLAN_CASES)
+def test_planner_hand_derived(case):
+    """The library's planner against the hand-derived plans (not against the
+    oracle's copy of the same code)."""
+    a = gt.plan_partition(case["dims"], case["nnz"], case["R"], case["mem"])
+    assert a["type"] == 0 and a["streams"] == 4
+    assert a["part"].tolist() == case["part"]
+    assert a["bdims"].tolist() == case["bd"]
+    assert a["avail_nnz"] == case["avail"]
+
+
 def test_planner_forced_and_invalid():
     a = gt.plan_partition([1000, 1000, 1000], 10**6, 10, 1 << 30, forced=[4, 4, 4])
     assert a["part"].tolist() == [4, 4, 4] and a["bdims"].tolist() == [250, 250, 250]

@@ -105,3 +105,41 @@ def test_cache_of_another_order_is_ignored(tmp_path):
 def test_new_flags_are_accepted():
     rc, out = run("-i", README_TNS, "-o", "3", "--parse-only", "--fused-fit", "--resume", "x")
     assert rc == 0, out
+
+
+def test_dump_init_follows_glibc_rand_order(tmp_path):
+    """The initial factors and core come from unseeded glibc rand() in the
+    reference's order: a value missing from the input is drawn first, while
+    parsing (tensor_manager.tpp:150-155); then every factor, axis by axis, row
+    by row over the whole mode (the parts are consecutive row ranges), column
+    by column (tucker.cuh:84-97); then the core in linear order
+    (tucker.cuh:118-119).  Each draw is frand<double>(0, 1) =
+    rand() / RAND_MAX in double (helper.hpp:99), stored as float.  Restated
+    here with libc's own rand() through ctypes."""
+    import ctypes
+    import numpy as np
+    p = tmp_path / "t.tns"
+    p.write_text("1\t1\t1\t1.5\n2\t3\t1\n3\t1\t2\t0.5\n")
+    prefix = str(tmp_path / "init")
+    rc, out = run("-i", str(p), "-o", "3", "-r", "4", "--dump-init", prefix)
+    assert rc == 0, out
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)  # the state of a process that never seeded (C standard: seed 1)
+    rand_max = 2147483647  # glibc RAND_MAX
+
+    def frand():
+        return np.float32(float(libc.rand()) / rand_max)
+
+    missing = frand()
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only")
+    assert rc == 0
+    ss = summary(out)[3]
+    assert abs(ss - (1.5 ** 2 + 0.5 ** 2 + float(missing) ** 2)) < 1e-6
+    dims, R = [3, 3, 2], 4
+    for a, d in enumerate(dims):
+        want = np.array([[frand() for _ in range(R)] for _ in range(d)], np.float32)
+        got = np.loadtxt(f"{prefix}_factor{a}.txt", ndmin=2).astype(np.float32)
+        assert np.array_equal(got, want), a
+    want = np.array([frand() for _ in range(R ** 3)], np.float32).reshape(-1, R)
+    got = np.loadtxt(f"{prefix}_core.txt", ndmin=2).astype(np.float32)
+    assert np.array_equal(got, want)

@@ -91,6 +91,43 @@ def test_delta_parity(N, R, part):
         assert err.max() < 2e-5 * max(1, R ** (N - 1) / 100), (m, err.max())
 
 
+def _stream_positions(t, idx_stream):
+    """Block-order position of each nonzero of a mode's stream, matched on
+    its global coordinates."""
+    radix = [t.rows(k) for k in range(t.order)]
+    g = t.global_rows().astype(np.int64)
+    kb = np.ravel_multi_index(tuple(g), radix)
+    ks = np.ravel_multi_index(tuple(idx_stream.astype(np.int64)), radix)
+    ob = np.argsort(kb)
+    pos = ob[np.searchsorted(kb[ob], ks)]
+    assert np.array_equal(kb[pos], ks)
+    return pos
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("bf16r", 1e-4), ("bf16", 2e-2)])
+@pytest.mark.parametrize("N,R,part", [(3, 10, [2, 1, 2]), (3, 13, [1, 1, 1]), (4, 8, [1, 2, 1, 1]),
+                                      (5, 5, [1, 1, 1, 2, 1]), (2, 6, [2, 1]), (3, 20, [1, 1, 1])])
+def test_hot_path_delta_parity(prec, tol, N, R, part):
+    """The deltas the mode update's own delta + Gram kernel forms (read back
+    through a test hook that stores each tile before its Gram), against the
+    FP64 oracle's delta (delta.cuh:62-90) of the same nonzero, relative to
+    the largest entry of that nonzero's delta.  Covers the FP32 VALU delta
+    (two tiles per pass, and the R > 15 kernel), the refined BF16 MFMA delta
+    (order 3 from the LDS table, with the second MFMA at R = 13, and orders 4
+    and 5 with the core operand from L2) and plain BF16."""
+    if R > 15 and prec != "fp32":
+        pytest.skip("the BF16 delta covers R <= 15")
+    dims = [90, 70, 50, 30, 20][:N]
+    e, t, A, G = _setup(N * 17 + R, dims, 8000, R, part, skew=1.2, precision=prec, chunk_tiles=1)
+    scale_n = max(1, R ** (N - 1) / 100)
+    for m in range(N):
+        d, idx_s = e.get_stream_delta(m)
+        ref = oracle.delta(t, A, G, m)[_stream_positions(t, idx_s)]
+        scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-30
+        err = (np.abs(d - ref) / scale).max()
+        assert err < tol * scale_n, (m, err)
+
+
 @pytest.mark.parametrize("chunk_tiles", [1, 2, 0])
 def test_gram_parity_heavy_rows(chunk_tiles):
     """Zipf rows (degrees up to thousands) with tiny chunks: many rows straddle
@@ -126,7 +163,19 @@ def test_mode_update_and_fit_parity(N, R, part, skew):
         B, Cv = oracle.bc(t, d, m, lam)
         A_ref[m] = oracle.solve(B, Cv)
         got = e.get_factor(m)
-        assert _rel(got, A_ref[m][: dims[m]]) < 1e-3, (m, _rel(got, A_ref[m][: dims[m]]))
+        # forward error: the FP32 delta's rounding (about 1e-7 relative for
+        # R^(N-1) = 100 terms, more for more) moves the exact FP64 solution by
+        # about 1e-6 here (the row systems have condition numbers up to 1e6),
+        # so 2e-5 per 100 terms leaves a margin of ten
+        fwd = _rel(got, A_ref[m][: dims[m]])
+        assert fwd < 2e-5 * max(1, R ** (N - 1) / 100), (m, fwd)
+        # backward error against the oracle's exact FP64 system, which the
+        # conditioning does not amplify: |B a - C| <= 1e-5 |C| per row
+        ag = np.zeros_like(A_ref[m])
+        ag[: dims[m]] = got
+        res = np.linalg.norm(np.einsum("rij,rj->ri", B, ag) - Cv, axis=1)
+        bwd = (res / (np.linalg.norm(Cv, axis=1) + 1e-300))[np.linalg.norm(Cv, axis=1) > 0]
+        assert bwd.max() < 1e-5, (m, bwd.max())
         e.set_factor(m, A_ref[m][: dims[m]].astype(np.float32))  # continue from the same point
     f = e.fit()
     xh = oracle.recon(t, [a.astype(np.float32).astype(np.float64) for a in A_ref], G)
@@ -146,7 +195,45 @@ def test_als_fit_trajectory_vs_oracle(part):
                         "f32")
     assert len(fits) == 5
     assert np.max(np.abs(fits - f64)) < 1e-5, (fits, f64)
-    assert np.max(np.abs(fits - f32)) < 1e-4, (fits, f32)
+    assert np.max(np.abs(fits - f32)) < 1e-5, (fits, f32)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16r"])
+def test_full_size_plain_config_vs_oracle(prec):
+    """The plain-method workload at its own size: 2000^3, 1e6 distinct uniform
+    nonzeros, U[0,1) values, R = 10, P = 1, 5 ALS iterations, lambda = 1e-3.
+    The FP32 path's fit in the reference's float arithmetic (ref_fit) must be
+    within 1e-5 of the reference-precision oracle (float delta, FP32 products,
+    serial float norm and error sums), and its FP64 fit within 1e-5 of the
+    FP64 oracle; the BF16-refined path's fit within 1e-4 relative of FP64.
+    At this size the two oracles themselves differ by 1.5e-5, all of it from
+    the serial float sums of the norm and the error."""
+    from gputucker_amd import synth
+    print(f"C1 full size: 2000^3, 1e6 nnz, R=10, P=1, 5 iterations, {prec}", flush=True)
+    idx, vals, dims = synth.generate([2000] * 3, 1_000_000, None, "uniform", seed=0)
+    A, G = synth.init_factors(dims, 10, seed=0)
+    t = oracle.Tensor(idx, vals.astype(np.float64), dims, [1, 1, 1], 10)
+    Ap = [oracle.pad_factor(a.astype(np.float64), t.rows(k)) for k, a in enumerate(A)]
+    f64, _ = oracle.als(t, Ap, G.astype(np.float64), 1e-3, 5, 0.0, "f64")
+    e = make(idx, vals, dims, 10, [1, 1, 1], precision=prec, lam=1e-3, max_iters=5, tol=0.0)
+    e.set_init(A, G)
+    fits = e.run()
+    assert len(fits) == 5
+    if prec == "bf16r":
+        rel = np.abs(fits - f64) / np.abs(f64)
+        assert rel.max() < 1e-4, (fits, f64)
+        return
+    assert np.max(np.abs(fits - f64)) < 1e-5, (fits, f64)
+    t32 = oracle.Tensor(idx, vals, dims, [1, 1, 1], 10)
+    f32, _ = oracle.als(t32, A, G, 1e-3, 5, 0.0, "f32")
+    er = make(idx, vals, dims, 10, [1, 1, 1], precision="fp32", lam=1e-3, max_iters=5, tol=0.0,
+              ref_fit=True)
+    er.set_init(A, G)
+    fits_ref = er.run()
+    assert np.max(np.abs(fits_ref - f32)) < 1e-5, (fits_ref, f32)
+    # the same factors as the FP64-sum run: only the fit's arithmetic differs
+    for m in range(3):
+        assert np.array_equal(er.get_factor(m), e.get_factor(m)), m
 
 
 def test_error_paths():
@@ -169,7 +256,7 @@ def test_error_paths():
 @pytest.mark.parametrize("N,R,part", [(3, 10, [2, 1, 2]), (4, 5, [1, 2, 1, 1]), (2, 8, [1, 3]),
                                       (3, 13, [2, 1, 2]), (3, 15, [1, 1, 1]), (3, 7, [1, 2, 1]),
                                       (2, 13, [2, 1]), (4, 7, [1, 2, 1, 1]), (4, 9, [1, 1, 2, 1]),
-                                      (5, 4, [1, 1, 1, 2, 1])])
+                                      (5, 4, [1, 1, 1, 2, 1]), (5, 5, [1, 1, 1, 2, 1])])
 def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
     """BF16 MFMA delta (plain and with the split-BF16 refinement; above
     R = 10 the refinement takes a second MFMA per tile; at order 4 with
@@ -331,6 +418,18 @@ def test_rank32_sample():
                              32, [1, 1, 1], "fp32", 6, 1e-5)
 
 
+def test_rank32_heavy_row_sample():
+    """The rank-32 workload's row skew: 3e5 nonzeros with Zipf(1.5) rows in
+    mode 0 (its largest row holds about 1.15e5 nonzeros, spread over some
+    1800 chunks, so the two-level fix-up sums it), R = 32, one ALS iteration
+    against the FP64 oracle."""
+    print("rank 32 heavy row: 5e4 x 5e4 x 5e3, 3e5 nnz, Zipf(1.5) in mode 0", flush=True)
+    e = _one_iteration_vs_oracle([50_000, 50_000, 5_000], 300_000,
+                                 [("zipf", 1.5), "uniform", "uniform"], "uniform", 32, [1, 1, 1],
+                                 "fp32", 6, 1e-5)
+    assert e.info()["nnz"] == 300_000
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16r"])
 def test_order5_out_of_core_sample(prec):
     """Shape of the 5-order rank-5 workload that runs under a device budget:
@@ -355,3 +454,29 @@ def test_order5_out_of_core_sample(prec):
     Ap = [oracle.pad_factor(a.astype(np.float64), t.rows(k)) for k, a in enumerate(A)]
     f64, _ = oracle.als(t, Ap, G.astype(np.float64), gt.default_lambda(), 1, 0.0, "f64")
     assert abs(f_oc[0] - f64[0]) / abs(f64[0]) < (1e-5 if prec == "fp32" else 1e-4)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16r"])
+def test_out_of_core_24bit_indices(prec):
+    """Mode 0 has 70000 rows (> 2^16), so the streams of modes 1 and 2 carry
+    its indices packed to 24 bits (four in three words) and the other index
+    in 16 bits: 3 + 2 + 4 = 9 host bytes per nonzero, the largest over the
+    modes.  Two staged segments; factors bit-identical to the in-core run."""
+    print(f"out of core 24-bit: 70000x60x50, 20000 nnz, R=4, {prec}", flush=True)
+    idx, vals, dims = random_coo(23, [70000, 60, 50], 20000)
+    A, G = random_init(23, dims, 4)
+    kw = dict(precision=prec, chunk_tiles=1, max_iters=2, tol=0.0)
+    ref = make(idx, vals, dims, 4, [1, 1, 1], **kw)
+    ref.set_init(A, G)
+    # per-row state (Gram rows, factors, row pointers) is about 10.6 MB here;
+    # 11.2 MB leaves room for two segments of about 11000 nonzeros
+    oc = make(idx, vals, dims, 4, [1, 1, 1], device_budget=11_200_000, **kw)
+    oc.set_init(A, G)
+    info = oc.info()
+    assert info["out_of_core"] == 1 and ref.info()["out_of_core"] == 0
+    assert info["stream_bytes_per_nnz"] == 9
+    assert -(-20000 // info["segment_nnz"]) >= 2
+    f_ref, f_oc = ref.run(), oc.run()
+    for m in range(3):
+        assert np.array_equal(ref.get_factor(m), oc.get_factor(m)), m
+    assert np.allclose(f_ref, f_oc, rtol=0, atol=1e-12)

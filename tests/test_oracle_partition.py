@@ -116,6 +116,53 @@ def test_plan_small_fits_in_memory():
     assert p["avail_nnz"] == ((64 << 30) - (core + sub)) // (12 + 4 + 40)
 
 
+# Hand-derived plans of the dimension-based branch (optimizer.tpp:137-153,
+# 202-376), worked step by step below with the reference's size model:
+#   all_data = avg*(4N + 4) + R^N*(4N + 4) + sum_n bd_n*R*4 + avg*R*4,
+#   avg = ceil(nnz / blocks); stop at the first P with all_data < mem/4 and
+#   blocks >= 4; each step increments the axis of the largest block_dims,
+#   ties to the lowest axis (optimizer.tpp:343-351);
+#   avail = (mem/4 - R^N*(4N + 4) - sum_n bd_n*R*4) / (4N + 4 + 4R).
+PLAN_CASES = [
+    # dims (1e5, 5e4, 2e3), 5e7 nnz, R = 10, mem = 256 MiB, mem/4 = 67108864.
+    # Steps (axis chosen): (2,1,1) 0; (3,1,1) 0 [tie 50000/50000]; (3,2,1) 1;
+    # (4,2,1) 0; (5,2,1) 0 [tie 25000]; (5,3,1) 1; (6,3,1) 0; (7,3,1) 0
+    # [tie 16667]; (7,4,1) 1; (8,4,1) 0; (9,4,1) 0 [tie 12500]:
+    #   36 blocks, avg 1388889: 77777784 + 16000 + (11112+12500+2000)*40
+    #   = 78818264, not below mem/4;
+    # (9,5,1) 1: 45 blocks, avg 1111112: 62222272 + 16000 + 924480 = 63162752 < mem/4.
+    # avail = (67108864 - 16000 - 924480) / 56 = 1181578.
+    dict(dims=[100000, 50000, 2000], nnz=50_000_000, R=10, mem=256 << 20,
+         part=[9, 5, 1], bd=[11112, 10000, 2000], avail=1181578),
+    # dims (4000, 4000, 10), 1e6 nnz, R = 4, mem = 24e6 (mem/4 = 6e6); the
+    # whole tensor needs 16e6 + 1024 + 128160 + 16e6 > mem: dimension-based.
+    # (2,1,1) 0 [tie 4000]: 2 blocks; (2,2,1) 1: avg 250000, 8000000 + 1024 +
+    # (2000+2000+10)*16 = 8065184; (3,2,1) 0 [tie 2000]: avg 166667, 5333344 +
+    # 1024 + (1334+2000+10)*16 = 5387872 < 6e6.  A tie broken to the highest
+    # axis would give (2,3,1).  avail = (6e6 - 1024 - 53504) / 32 = 185796.
+    dict(dims=[4000, 4000, 10], nnz=1_000_000, R=4, mem=24_000_000,
+         part=[3, 2, 1], bd=[1334, 2000, 10], avail=185796),
+    # order 4: dims (3000, 3000, 2000, 5), 2e6 nnz, R = 6, mem = 36e6
+    # (mem/4 = 9e6); the whole tensor needs 2e6*44 + 1296*20 + 8005*24 =
+    # 88218040 > mem.  (2,1,1,1) 0 [tie 3000]; (2,2,1,1) 1; (2,2,2,1) 2:
+    # avg 250000, 11000000 + 25920 + (1500+1500+1000+5)*24 = 11122040;
+    # (3,2,2,1) 0 [tie 1500]: avg 166667, 7333348 + 25920 + 3505*24 = 7443388
+    # < 9e6.  Ties to the highest axis would give (2,3,2,1).
+    # avail = (9e6 - 25920 - 84120) / 44 = 202044.
+    dict(dims=[3000, 3000, 2000, 5], nnz=2_000_000, R=6, mem=36_000_000,
+         part=[3, 2, 2, 1], bd=[1000, 1500, 1000, 5], avail=202044),
+]
+
+
+@pytest.mark.parametrize("case", PLAN_CASES)
+def test_plan_dimension_branch_hand_derived(case):
+    p = oracle.plan(case["dims"], case["nnz"], case["R"], 1, case["mem"])
+    assert p["type"] == 0 and p["streams"] == 4
+    assert p["part"].tolist() == case["part"]
+    assert p["bdims"].tolist() == case["bd"]
+    assert p["avail_nnz"] == case["avail"]
+
+
 def test_plan_dimension_partitioning_invariants():
     dims, nnz, R, mem = [100000, 50000, 2000], 50_000_000, 10, 256 << 20
     p = oracle.plan(dims, nnz, R, 1, mem)
