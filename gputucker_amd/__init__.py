@@ -27,8 +27,8 @@ STATUS = {0: "GT_OK", 1: "GT_ERR_ARG", 2: "GT_ERR_HIP", 3: "GT_ERR_STATE",
 EXPORTS = [
     "gt_default_opts", "gt_plan_partition", "gt_schedule", "gt_create", "gt_destroy",
     "gt_last_error", "gt_set_coo", "gt_partition", "gt_get_partition_info", "gt_get_block_nnz", "gt_get_block",
-    "gt_partition_dump", "gt_set_init", "gt_mode_update", "gt_fit", "gt_iteration",
-    "gt_run",
+    "gt_partition_dump", "gt_set_init", "gt_mode_update", "gt_fit", "gt_fit_noalloc",
+    "gt_iteration", "gt_run",
     "gt_get_factor", "gt_set_factor", "gt_get_delta", "gt_get_stream_delta", "gt_get_gram",
     "gt_get_recon",
     "gt_get_timings", "gt_synchronize", "gt_version",
@@ -102,6 +102,7 @@ def lib():
         L.gt_set_init.argtypes = [vp, C.POINTER(vp), vp]
         L.gt_mode_update.argtypes = [vp, C.c_int]
         L.gt_fit.argtypes = [vp, C.POINTER(C.c_double)]
+        L.gt_fit_noalloc.argtypes = [vp, C.POINTER(C.c_double)]
         L.gt_iteration.argtypes = [vp, C.POINTER(C.c_double)]
         L.gt_run.argtypes = [vp, C.POINTER(C.c_int), vp]
         L.gt_get_factor.argtypes = [vp, C.c_int, vp]
@@ -268,6 +269,12 @@ class Engine:
     def fit(self):
         f = C.c_double()
         self._check(lib().gt_fit(self._h, C.byref(f)))
+        return f.value
+
+    def fit_noalloc(self):
+        """gt_fit through the entry point that allocates no device memory."""
+        f = C.c_double()
+        self._check(lib().gt_fit_noalloc(self._h, C.byref(f)))
         return f.value
 
     def iteration(self):

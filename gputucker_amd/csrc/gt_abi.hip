@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -32,7 +33,32 @@
 
 using namespace gt;
 
+// Scratch of gt_fit_noalloc (the error partials of the reconstruction pass
+// and their sum): a file-scope device array, so that entry point allocates
+// nothing.  Each device the library runs on holds its own copy until the
+// process ends.  On the host `scratch` is only a stand-in for the symbol:
+// gt_create looks up the device's address once per device with
+// hipGetSymbolAddress, and host code uses that address, never the name.
+constexpr size_t kScratchDoubles = 16384; // 128 KiB
+__device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
+
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_at; // per device ordinal, null until looked up
+
+// The device address of `scratch` on device `dev`, which must be current.
+double *scratch_address(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_at.size() <= dev) g_scratch_at.resize(dev + 1, nullptr);
+  if (!g_scratch_at[dev]) {
+    void *p = nullptr;
+    GT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_at[dev] = static_cast<double *>(p);
+  }
+  return g_scratch_at[dev];
+}
 
 thread_local std::string g_create_error;
 
@@ -221,6 +247,7 @@ struct gt_ctx {
   hipStream_t s_copy = nullptr;
   hipEvent_t ev_copy[2] = {}, ev_comp[2] = {};
   // timing
+  double *scratch = nullptr; // this device's address of the static scratch array
   hipEvent_t ev[8] = {};
   hipEvent_t ev_mode[GT_MAX_ORDER][3] = {}; // per mode: start, Gram done, solve done
   bool mode_pending[GT_MAX_ORDER] = {};     // recorded, not yet read into tm
@@ -1138,6 +1165,7 @@ int gt_create(gt_ctx **out, const gt_opts *opts) {
     GT_HIP(hipGetDeviceCount(&count));
     if (opts->device < 0 || opts->device >= count) throw GtError(GT_ERR_ARG, "no such device");
     GT_HIP(hipSetDevice(opts->device));
+    c->scratch = scratch_address(opts->device);
     GT_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto &e : c->ev) GT_HIP(hipEventCreate(&e));
     for (auto &em : c->ev_mode)
@@ -1344,6 +1372,21 @@ int gt_mode_update(gt_ctx *ctx, int mode) {
 int gt_fit(gt_ctx *ctx, double *fit) {
   return guarded(ctx, [&] {
     double f = ctx->o.ref_fit ? compute_fit_ref(ctx) : compute_fit(ctx, ctx->err_part.p, ctx->scalar.p);
+    if (fit) *fit = f;
+  });
+}
+
+int gt_fit_noalloc(gt_ctx *ctx, double *fit) {
+  return guarded(ctx, [&] {
+    require(ctx->partitioned && ctx->initialized, GT_ERR_STATE, "partition and init first");
+    const uint64_t need = fit_partials_needed(ctx) + 1;
+    if (need > kScratchDoubles)
+      throw GtError(GT_ERR_MEMORY,
+                    "gt_fit_noalloc needs " + std::to_string(need) +
+                        " doubles of scratch (2048 per staged segment, plus one); the static "
+                        "scratch holds " + std::to_string(kScratchDoubles) +
+                        ": use gt_fit for this out-of-core layout");
+    double f = compute_fit(ctx, ctx->scratch, ctx->scratch + (kScratchDoubles - 1));
     if (fit) *fit = f;
   });
 }

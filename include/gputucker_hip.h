@@ -166,6 +166,17 @@ int gt_mode_update(gt_ctx *ctx, int mode);
 /* Replaces Reconstruction (reconstruction.cuh:228-277): fit = 1 - ||x - xhat|| / ||x||. */
 int gt_fit(gt_ctx *ctx, double *fit);
 
+/* gt_fit without any device allocation.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under stream capture, tight loops, harnesses that forbid hipMalloc).  The
+ * error partials and their sum live in a static 128 KiB device array (16384
+ * doubles), one per device for the life of the process, shared by every
+ * context on that device: calls that overlap on one device race.  It needs
+ * 2048 doubles per staged segment plus one (in core: one segment), and
+ * returns GT_ERR_MEMORY for an out-of-core layout of more than 7 segments.
+ * Same kernels and summation order as gt_fit, so the same result. */
+int gt_fit_noalloc(gt_ctx *ctx, double *fit);
+
 /* One ALS iteration, modes 0..N-1 then the fit (update.cuh:146-198 + tucker.cuh:169-176). */
 int gt_iteration(gt_ctx *ctx, double *fit);
 
