@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(_HERE, "libgputucker_hip.so")
 MAX_ORDER = 8
 
 PRECISIONS = {"fp32": 0, "bf16": 1, "bf16r": 2}
+# gt_delta_path: the delta + Gram kernel a mode update ran
+DELTA_PATHS = {0: "none", 1: "fp32", 2: "fp32-wide", 3: "bf16", 4: "bf16r", 5: "bf16r-l2"}
 STATUS = {0: "GT_OK", 1: "GT_ERR_ARG", 2: "GT_ERR_HIP", 3: "GT_ERR_STATE",
           4: "GT_ERR_UNSUPPORTED", 5: "GT_ERR_MEMORY", 6: "GT_ERR_PARTITION"}
 
@@ -61,7 +63,7 @@ class GtTimings(C.Structure):
         ("ms_gram", C.c_double * MAX_ORDER), ("ms_solve", C.c_double * MAX_ORDER),
         ("ms_mode", C.c_double * MAX_ORDER), ("ms_fit", C.c_double),
         ("ms_partition", C.c_double), ("gram_kernel_ms_total", C.c_double),
-        ("gram_kernel_launches", C.c_uint64),
+        ("gram_kernel_launches", C.c_uint64), ("delta_path", C.c_int * MAX_ORDER),
     ]
 
 
@@ -329,7 +331,8 @@ class Engine:
         return dict(ms_gram=list(t.ms_gram[:N]), ms_solve=list(t.ms_solve[:N]),
                     ms_mode=list(t.ms_mode[:N]), ms_fit=t.ms_fit, ms_partition=t.ms_partition,
                     gram_kernel_ms_total=t.gram_kernel_ms_total,
-                    gram_kernel_launches=t.gram_kernel_launches)
+                    gram_kernel_launches=t.gram_kernel_launches,
+                    delta_path=[DELTA_PATHS[p] for p in t.delta_path[:N]])
 
     def synchronize(self):
         self._check(lib().gt_synchronize(self._h))

@@ -188,7 +188,9 @@ bool plan_partition(int N, const uint32_t *dims, uint64_t nnz, int rank, uint64_
 struct gt_ctx {
   gt_opts o{};
   std::string err;
-  int N = 0, R = 0;
+  int N = 0;
+  int R = 0;  // the rank the kernels run at (kernel_rank of Ru)
+  int Ru = 0; // the model's rank: host arrays at the boundary have Ru columns
   hipStream_t s = nullptr;
   // input
   uint64_t nnz = 0;
@@ -372,6 +374,23 @@ uint64_t ipow(uint64_t b, int e) {
   uint64_t v = 1;
   for (int i = 0; i < e; ++i) v *= b;
   return v;
+}
+
+// The Ru^N core (linear order, last axis fastest) embedded in the RK^N core
+// of the kernel rank, zero where any coordinate is Ru or above.
+std::vector<float> pad_core(const float *core, int N, int Ru, int RK) {
+  const uint64_t total = ipow(Ru, N);
+  std::vector<float> out(ipow(RK, N), 0.f);
+  for (uint64_t i = 0; i < total; ++i) {
+    uint64_t rem = i, at = 0, mult = 1;
+    for (int a = N - 1; a >= 0; --a) {
+      at += (rem % Ru) * mult;
+      rem /= Ru;
+      mult *= RK;
+    }
+    out[at] = core[i];
+  }
+  return out;
 }
 
 // Core unfolding for mode n in the [o][jL][r] order the kernels read.
@@ -640,10 +659,14 @@ void launch_gram_window(gt_ctx *c, const GramArgs &a) {
     e1 = c->get_event();
     GT_HIP(hipEventRecord(e0, c->s));
   }
-  if (gram_mfma_supported(c->R))
-    launch_gram_mfma(c->R, c->o.precision, a, c->O, c->s);
-  else
+  int path;
+  if (gram_mfma_supported(c->R)) {
+    path = launch_gram_mfma(c->R, c->o.precision, a, c->O, c->s);
+  } else {
     launch_gram_fp32(c->R, a, c->s);
+    path = GT_DELTA_FP32_WIDE;
+  }
+  c->tm.delta_path[a.mode] = path;
   if (c->o.profile) {
     GT_HIP(hipEventRecord(e1, c->s));
     c->prof_pending.emplace_back(e0, e1);
@@ -742,11 +765,12 @@ uint64_t mode_update(gt_ctx *c, int n, bool fit_partials, bool sync) {
   return parts;
 }
 
-// I_n x R host rows into the padded device layout (one contiguous copy).
+// I_n x Ru host rows into the padded device layout (one contiguous copy);
+// columns Ru..RS-1 stay zero.
 void upload_factor(gt_ctx *c, int n, const float *src) {
-  const int R = c->R, RS = row_stride(R);
+  const int Ru = c->Ru, RS = row_stride(c->R);
   std::vector<float> pad((size_t)c->dims[n] * RS, 0.f);
-  for (uint64_t i = 0; i < c->dims[n]; ++i) std::memcpy(pad.data() + i * RS, src + i * R, R * 4);
+  for (uint64_t i = 0; i < c->dims[n]; ++i) std::memcpy(pad.data() + i * RS, src + i * Ru, Ru * 4);
   GT_HIP(hipMemcpyAsync(c->A[n].p, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice,
                         c->s));
   refresh_split(c, n);
@@ -772,6 +796,21 @@ CooArgs coo_args(gt_ctx *c, int mode) {
   a.val = c->coo_val.p;
   a.Gt = c->Gt[mode].p;
   return a;
+}
+
+// nnz x R deltas of the kernel rank (device) to nnz x Ru on the host.
+void copy_delta_out(gt_ctx *c, const float *d, float *out) {
+  const uint64_t nnz = c->nnz;
+  if (c->Ru == c->R) {
+    GT_HIP(hipMemcpyAsync(out, d, nnz * c->R * sizeof(float), hipMemcpyDeviceToHost, c->s));
+    GT_HIP(hipStreamSynchronize(c->s));
+    return;
+  }
+  std::vector<float> h(nnz * c->R);
+  GT_HIP(hipMemcpyAsync(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->s));
+  GT_HIP(hipStreamSynchronize(c->s));
+  for (uint64_t e = 0; e < nnz; ++e)
+    std::memcpy(out + e * c->Ru, h.data() + e * c->R, c->Ru * sizeof(float));
 }
 
 // FP64 partials the reconstruction pass of compute_fit writes: one grid's
@@ -934,7 +973,7 @@ void do_partition(gt_ctx *c) {
     mem = fr;
   }
   Plan p;
-  if (!plan_partition(N, c->dims, nnz, c->R, mem, (int)sizeof(float), c->o.partition, p))
+  if (!plan_partition(N, c->dims, nnz, c->Ru, mem, (int)sizeof(float), c->o.partition, p))
     throw GtError(GT_ERR_PARTITION, "Block dimension is larger than the tensor dimension.");
   require(p.blocks < (1ull << 32), GT_ERR_ARG, "too many blocks");
   c->plan = p;
@@ -1134,8 +1173,9 @@ int gt_create(gt_ctx **out, const gt_opts *opts) {
     g_create_error = "order must be in [2, 8]";
     return GT_ERR_UNSUPPORTED;
   }
-  if (opts->rank < 1 || opts->rank > GT_MAX_RANK || !rank_supported(opts->rank)) {
-    g_create_error = "rank " + std::to_string(opts->rank) + " is not built";
+  if (opts->rank < 1 || opts->rank > GT_MAX_RANK || kernel_rank(opts->rank) == 0) {
+    g_create_error = "rank " + std::to_string(opts->rank) + " is outside 1.." +
+                     std::to_string(GT_MAX_RANK);
     return GT_ERR_UNSUPPORTED;
   }
   if (!(opts->lambda > 0.0)) {
@@ -1150,7 +1190,7 @@ int gt_create(gt_ctx **out, const gt_opts *opts) {
     g_create_error = "the BF16 MFMA delta is built for rank <= 15";
     return GT_ERR_UNSUPPORTED;
   }
-  if (ipow(opts->rank, opts->order) > (1ull << 24)) {
+  if (ipow(kernel_rank(opts->rank), opts->order) > (1ull << 24)) {
     g_create_error = "core too large";
     return GT_ERR_UNSUPPORTED;
   }
@@ -1158,7 +1198,8 @@ int gt_create(gt_ctx **out, const gt_opts *opts) {
   c->o = *opts;
   if (c->o.max_iters <= 0) c->o.max_iters = 3;
   c->N = opts->order;
-  c->R = opts->rank;
+  c->Ru = opts->rank;
+  c->R = kernel_rank(opts->rank);
   c->O = (int)ipow(c->R, c->N - 2);
   try {
     int count = 0;
@@ -1166,6 +1207,9 @@ int gt_create(gt_ctx **out, const gt_opts *opts) {
     if (opts->device < 0 || opts->device >= count) throw GtError(GT_ERR_ARG, "no such device");
     GT_HIP(hipSetDevice(opts->device));
     c->scratch = scratch_address(opts->device);
+    int lds = 0;
+    GT_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, opts->device));
+    set_lds_limit((size_t)lds);
     GT_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     for (auto &e : c->ev) GT_HIP(hipEventCreate(&e));
     for (auto &em : c->ev_mode)
@@ -1343,13 +1387,14 @@ int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core) {
     require(ctx->partitioned, GT_ERR_STATE, "partition first");
     require(A && core, GT_ERR_ARG, "null argument");
     const int N = ctx->N, R = ctx->R;
+    const std::vector<float> core_k = pad_core(core, N, ctx->Ru, R);
     for (int n = 0; n < N; ++n) {
       require(A[n] != nullptr, GT_ERR_ARG, "null factor");
       const int RS = row_stride(R);
       ctx->A[n].alloc(ctx->rows[n] * RS);
       GT_HIP(hipMemsetAsync(ctx->A[n].p, 0, ctx->rows[n] * RS * sizeof(float), ctx->s));
       upload_factor(ctx, n, A[n]);
-      std::vector<float> gt = unfold_core(core, N, R, n);
+      std::vector<float> gt = unfold_core(core_k.data(), N, R, n);
       ctx->Gt[n].alloc(gt.size());
       GT_HIP(hipMemcpyAsync(ctx->Gt[n].p, gt.data(), gt.size() * sizeof(float),
                             hipMemcpyHostToDevice, ctx->s));
@@ -1418,13 +1463,13 @@ int gt_get_factor(gt_ctx *ctx, int mode, float *out) {
   return guarded(ctx, [&] {
     require(ctx->initialized, GT_ERR_STATE, "not initialized");
     require(mode >= 0 && mode < ctx->N && out, GT_ERR_ARG, "bad argument");
-    const int R = ctx->R, RS = row_stride(R);
+    const int Ru = ctx->Ru, RS = row_stride(ctx->R);
     std::vector<float> pad((size_t)ctx->dims[mode] * RS);
     GT_HIP(hipMemcpyAsync(pad.data(), ctx->A[mode].p, pad.size() * sizeof(float),
                           hipMemcpyDeviceToHost, ctx->s));
     GT_HIP(hipStreamSynchronize(ctx->s));
     for (uint64_t i = 0; i < ctx->dims[mode]; ++i)
-      std::memcpy(out + i * R, pad.data() + i * RS, R * sizeof(float));
+      std::memcpy(out + i * Ru, pad.data() + i * RS, Ru * sizeof(float));
   });
 }
 
@@ -1442,13 +1487,11 @@ int gt_get_delta(gt_ctx *ctx, int mode, float *out) {
     require(!ctx->streaming, GT_ERR_STATE, "the block COO is not resident (out-of-core mode)");
     require(mode >= 0 && mode < ctx->N && out, GT_ERR_ARG, "bad argument");
     DevBuf<float> d;
-    d.alloc(ctx->nnz * ctx->R);
+    d.alloc(std::max<uint64_t>(ctx->nnz * ctx->R, 1));
     CooArgs a = coo_args(ctx, mode);
     a.delta_out = d.p;
     launch_coo_contract(ctx->R, a, ctx->s);
-    GT_HIP(hipMemcpyAsync(out, d.p, ctx->nnz * ctx->R * sizeof(float), hipMemcpyDeviceToHost,
-                          ctx->s));
-    GT_HIP(hipStreamSynchronize(ctx->s));
+    copy_delta_out(ctx, d.p, out);
   });
 }
 
@@ -1467,8 +1510,7 @@ int gt_get_stream_delta(gt_ctx *ctx, int mode, float *delta, uint32_t *const *id
       throw;
     }
     ctx->dbg_delta = nullptr;
-    GT_HIP(hipMemcpyAsync(delta, d.p, ctx->nnz * ctx->R * sizeof(float), hipMemcpyDeviceToHost,
-                          ctx->s));
+    copy_delta_out(ctx, d.p, delta);
     for (int k = 0; idx && k < ctx->N; ++k)
       if (idx[k])
         GT_HIP(hipMemcpyAsync(idx[k], ctx->st_idx[mode][k].p, ctx->nnz * sizeof(uint32_t),
@@ -1497,7 +1539,7 @@ int gt_get_gram(gt_ctx *ctx, int mode, double *B, double *C) {
     require(ctx->partitioned && ctx->initialized, GT_ERR_STATE, "partition and init first");
     require(mode >= 0 && mode < ctx->N && B && C, GT_ERR_ARG, "bad argument");
     run_gram(ctx, mode);
-    const int R = ctx->R, W = gram_width(R);
+    const int R = ctx->R, Ru = ctx->Ru, W = gram_width(R);
     const uint64_t rows = ctx->rows[mode];
     std::vector<double> g(rows * W);
     std::vector<uint64_t> rp(rows + 1);
@@ -1509,14 +1551,15 @@ int gt_get_gram(gt_ctx *ctx, int mode, double *B, double *C) {
     for (uint64_t r = 0; r < rows; ++r) {
       const bool empty = rp[r + 1] == rp[r];
       const double *gr = g.data() + r * W;
-      int q = 0;
+      int q = 0; // packed upper triangle of the kernel rank; the first Ru rows and columns out
       for (int p = 0; p < R; ++p)
         for (int p2 = p; p2 < R; ++p2, ++q) {
+          if (p2 >= Ru) continue;
           const double v = empty ? 0.0 : gr[q];
-          B[(r * R + p) * R + p2] = v + (p == p2 ? ctx->o.lambda : 0.0);
-          B[(r * R + p2) * R + p] = v + (p == p2 ? ctx->o.lambda : 0.0);
+          B[(r * Ru + p) * Ru + p2] = v + (p == p2 ? ctx->o.lambda : 0.0);
+          B[(r * Ru + p2) * Ru + p] = v + (p == p2 ? ctx->o.lambda : 0.0);
         }
-      for (int p = 0; p < R; ++p) C[r * R + p] = empty ? 0.0 : gr[R * (R + 1) / 2 + p];
+      for (int p = 0; p < Ru; ++p) C[r * Ru + p] = empty ? 0.0 : gr[R * (R + 1) / 2 + p];
     }
   });
 }

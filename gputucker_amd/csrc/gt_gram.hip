@@ -14,6 +14,8 @@
 // order, flushing its FP64 accumulator whenever the row changes.  delta never
 // reaches HBM.  Products of the FP32 deltas are formed exactly in FP64 and
 // summed in FP64 (the reference rounds each product to value_t first).
+#include <atomic>
+
 #include "gt_contract.hpp"
 
 namespace gt {
@@ -26,18 +28,41 @@ template <int R> struct GramLayout {
   static constexpr int RP = (R % 2 == 0) ? R + 1 : R + 2; // odd tile pitch; column R holds x
 };
 
+// LDS floats per wave of k_gram_fp32: the delta tile (kWave x RP, column R
+// holding x), the tile's row ids, and the rows contract_slots stages.
+template <int R> constexpr size_t fp32_lds_floats_per_wave(int slots) {
+  return (size_t)kWave * GramLayout<R>::RP + kWave + (size_t)contract_lds_rows<R>(slots) * R * kWave;
+}
+
+// Waves per workgroup for a kernel whose waves each take `per_wave` bytes of
+// LDS: the most waves per CU that the device's LDS holds, in workgroups of
+// 4, 2 or 1 waves (ties to the larger workgroup); 0 if not even one fits.
+int waves_per_block(size_t per_wave) {
+  const size_t limit = lds_limit();
+  int best = 0, best_waves = 0;
+  for (int w : {4, 2, 1}) {
+    const size_t wg = per_wave * w;
+    if (wg > limit) continue;
+    const int waves = (int)(limit / wg) * w;
+    if (waves > best_waves) {
+      best = w;
+      best_waves = waves;
+    }
+  }
+  return best;
+}
+
 template <int R>
-__global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
+__global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a, int wpb) {
   using Lay = GramLayout<R>;
   constexpr int W = Lay::W, NE = Lay::NE, E = Lay::E, RP = Lay::RP;
-  __shared__ float s_tile[4][kWave * RP];
-  __shared__ uint32_t s_row[4][kWave];
-  extern __shared__ float s_rows_dyn[];
+  extern __shared__ float s_dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
-  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * 4 + w;
+  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
   if (c >= a.chunk_end) return;
-  float *lrows = s_rows_dyn + (size_t)w * contract_lds_rows<R>(a.slots) * R * kWave;
-  float *tile = s_tile[w];
+  float *tile = s_dyn + (size_t)w * fp32_lds_floats_per_wave<R>(a.slots);
+  uint32_t *s_row = reinterpret_cast<uint32_t *>(tile + kWave * RP);
+  float *lrows = tile + kWave * RP + kWave;
   const uint64_t cb = c * a.chunk;
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
   const uint32_t *__restrict__ rows = a.idx_row;
@@ -103,7 +128,7 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
 #pragma unroll
     for (int r = 0; r < R; ++r) tile[lane * RP + r] = d[r];
     tile[lane * RP + R] = x;
-    s_row[w][lane] = rows[ee];
+    s_row[lane] = rows[ee];
     wave_sync();
     const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
     if (a.delta_out != nullptr && lane < cnt) { // test hook, uniform branch
@@ -111,7 +136,7 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a) {
       for (int r = 0; r < R; ++r) a.delta_out[(tb + lane) * R + r] = tile[lane * RP + r];
     }
     for (int j = 0; j < cnt; ++j) {
-      const uint32_t row = __builtin_amdgcn_readfirstlane(s_row[w][j]);
+      const uint32_t row = __builtin_amdgcn_readfirstlane(s_row[j]);
       if (row != cur) {
         flush(cur, run_begin, tb + j);
         cur = row;
@@ -268,7 +293,7 @@ __global__ void __launch_bounds__(kWave * kFixWaves) k_gram_fix_rows(GramArgs a)
 
 template <int R>
 __global__ void __launch_bounds__(256) k_coo_contract(CooArgs a) {
-  __shared__ double red[256];
+  __shared__ double red[256]; // blockDim.x entries: 64, 128 or 256
   extern __shared__ float s_rows_dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   float *lrows = s_rows_dyn + (size_t)w * contract_lds_rows<R>(a.slots) * R * kWave;
@@ -303,7 +328,7 @@ __global__ void __launch_bounds__(256) k_coo_contract(CooArgs a) {
   if (a.err_partials) {
     red[threadIdx.x] = err;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
       if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
       __syncthreads();
     }
@@ -331,9 +356,13 @@ __global__ void k_reduce_f64(const double *__restrict__ in, int n, double *__res
 
 template <int R> void gram_fp32_impl(const GramArgs &a, hipStream_t s) {
   if (a.chunk_end <= a.chunk_begin) return;
-  const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + 3) / 4);
-  const size_t lds = (size_t)4 * contract_lds_rows<R>(a.slots) * R * kWave * sizeof(float);
-  hipLaunchKernelGGL(k_gram_fp32<R>, dim3(grid), dim3(256), lds, s, a);
+  const size_t per_wave = fp32_lds_floats_per_wave<R>(a.slots) * sizeof(float);
+  const int wpb = waves_per_block(per_wave);
+  if (wpb == 0)
+    throw std::invalid_argument("FP32 Gram: one wave's LDS (" + std::to_string(per_wave) +
+                                " bytes) exceeds the device's LDS per workgroup");
+  const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
+  hipLaunchKernelGGL(k_gram_fp32<R>, dim3(grid), dim3(64 * wpb), per_wave * wpb, s, a, wpb);
 }
 
 template <int R> void gram_fixup_impl(const GramArgs &a, hipStream_t s) {
@@ -344,15 +373,27 @@ template <int R> void gram_fixup_impl(const GramArgs &a, hipStream_t s) {
 }
 
 template <int R> int coo_impl(const CooArgs &a, hipStream_t s) {
-  uint64_t g = (a.nnz + 255) / 256;
+  const size_t per_wave = (size_t)contract_lds_rows<R>(a.slots) * R * kWave * sizeof(float);
+  const int wpb = per_wave == 0 ? 4 : waves_per_block(per_wave + 2048 / 4);
+  if (wpb == 0)
+    throw std::invalid_argument("reconstruction: one wave's LDS exceeds the device's LDS");
+  const uint64_t threads = 64 * (uint64_t)wpb;
+  uint64_t g = (a.nnz + threads - 1) / threads;
   if (g < 1) g = 1;
   if (g > 2048) g = 2048;
-  const size_t lds = (size_t)4 * contract_lds_rows<R>(a.slots) * R * kWave * sizeof(float);
-  hipLaunchKernelGGL(k_coo_contract<R>, dim3((unsigned)g), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(k_coo_contract<R>, dim3((unsigned)g), dim3((unsigned)threads),
+                     per_wave * wpb, s, a);
   return (int)g;
 }
 
+std::atomic<size_t> g_lds_limit{64 * 1024};
+
 } // namespace
+
+size_t lds_limit() { return g_lds_limit.load(std::memory_order_relaxed); }
+void set_lds_limit(size_t bytes) {
+  if (bytes) g_lds_limit.store(bytes, std::memory_order_relaxed);
+}
 
 void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s) {
   switch (R) {

@@ -72,11 +72,14 @@ template <int R> __device__ inline int packed_index(int p, int q) {
   return -1;
 }
 
-// LDS one workgroup may take (a launch without attributes), and the static
-// LDS of the Gram kernels (delta tiles and row ids of four waves; the FP32
-// kernel keeps two of each per wave), which the dynamic part must leave room
-// for.
-constexpr size_t kLdsLimit = 64 * 1024;
+// The static LDS of the Gram kernels (delta tiles and row ids of four waves;
+// the FP32 kernel keeps two of each per wave), which the dynamic part must
+// leave room for within the device's LDS per workgroup (lds_limit(): 160 KiB
+// on gfx950).  A kernel variant that keeps its core table in LDS is chosen
+// only when the LDS still holds kMinLdsWavesPerCu waves per CU: below that
+// the waves cannot hide the gathers' latency, and the variant that reads the
+// core operand from L2 at three waves per SIMD is faster.
+constexpr int kMinLdsWavesPerCu = 8;
 constexpr size_t kStaticLdsValu = 4 * 2 * (16 * kDP * sizeof(float) + kWave * sizeof(uint32_t));
 constexpr size_t kStaticLdsBf16 = 4 * (16 * kDP * sizeof(float) + kWave * sizeof(uint32_t));
 
@@ -776,60 +779,74 @@ __global__ void __launch_bounds__(256) k_split_factor(const float *__restrict__ 
 template <int R> void launch_valu(const GramArgs &a, hipStream_t st) {
   const size_t per_wave = lds_floats_per_wave<kValu>(a.slots, R, 1) * sizeof(float);
   int wpb = 4;
-  while (wpb > 1 && per_wave * wpb + kStaticLdsValu > kLdsLimit) --wpb;
-  if (per_wave * wpb + kStaticLdsValu > kLdsLimit)
+  while (wpb > 1 && per_wave * wpb + kStaticLdsValu > lds_limit()) --wpb;
+  if (per_wave * wpb + kStaticLdsValu > lds_limit())
     throw std::runtime_error("FP32 Gram: outer factor rows do not fit in LDS");
+  if (a.chunk_end <= a.chunk_begin) return;
   const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
   hipLaunchKernelGGL((k_gram_valu<R>), dim3(grid), dim3(64 * wpb), per_wave * wpb, st, a, wpb);
 }
 
-template <int R, int KIND> void launch_impl(const GramArgs &a, int O, hipStream_t st) {
-  if (a.chunk_end <= a.chunk_begin) return;
+// Picks the delta + Gram kernel for rank R and the requested delta kind, and
+// launches it over the chunks of `a` (if any); returns the gt_delta_path
+// that ran.  A BF16 request falls back to the FP32 delta only where no MFMA
+// variant is built (plain BF16 whose table leaves too few waves per CU,
+// refined BF16 at orders of 6 and up or order 5 above R = 8); the returned
+// path says so.
+template <int R, int KIND> int launch_impl(const GramArgs &a, int O, hipStream_t st) {
+  const bool any = a.chunk_end > a.chunk_begin;
   if constexpr (KIND == kValu) {
     launch_valu<R>(a, st);
+    return GT_DELTA_FP32;
   } else {
     const size_t per_wave = lds_floats_per_wave<KIND>(a.slots, R, O) * sizeof(float);
     const size_t table =
         KIND == kBf16 ? gb_k16_words(O) * 2
                       : (gb_k32_words(O) * (bf16r_single_mfma(R) ? 1 : 2)) * 2;
-    if (per_wave + table + kStaticLdsBf16 > kLdsLimit || a.As_L == nullptr) { // tables too large
+    // waves per CU that the LDS holds with the core table in it, 4 per workgroup
+    const size_t wg4 = per_wave * 4 + table + kStaticLdsBf16;
+    const bool table_fits =
+        wg4 <= lds_limit() && (int)(lds_limit() / wg4) * 4 >= kMinLdsWavesPerCu;
+    if (!table_fits || a.As_L == nullptr) { // the core operand from L2 instead
       if constexpr (KIND == kBf16R && bf16r_single_mfma(R) && R >= 2) {
         // orders 4 and 5: the core operand from L2, the outer rows in LDS
         const int wpb = 4;
         const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
         const size_t rows_lds = (size_t)a.slots * R * kWave * sizeof(float);
-        if (a.As_L != nullptr && wpb * rows_lds + kStaticLdsBf16 <= kLdsLimit) {
+        if (a.As_L != nullptr && wpb * rows_lds + kStaticLdsBf16 <= lds_limit()) {
           if (a.slots == 2 && O == R * R) {
-            hipLaunchKernelGGL((k_gram_bf16_o4<R, 2>), dim3(grid), dim3(64 * wpb),
-                               rows_lds * wpb, st, a, wpb);
-            return;
+            if (any)
+              hipLaunchKernelGGL((k_gram_bf16_o4<R, 2>), dim3(grid), dim3(64 * wpb),
+                                 rows_lds * wpb, st, a, wpb);
+            return GT_DELTA_BF16R_L2;
           }
           if constexpr (bf16r_pair_table(R)) {
             if (a.slots == 3 && O == R * R * R) {
-              hipLaunchKernelGGL((k_gram_bf16_o4<R, 3>), dim3(grid), dim3(64 * wpb),
-                                 rows_lds * wpb, st, a, wpb);
-              return;
+              if (any)
+                hipLaunchKernelGGL((k_gram_bf16_o4<R, 3>), dim3(grid), dim3(64 * wpb),
+                                   rows_lds * wpb, st, a, wpb);
+              return GT_DELTA_BF16R_L2;
             }
           }
         }
       }
       launch_valu<R>(a, st);
-      return;
+      return GT_DELTA_FP32;
     }
-    int wpb = 4;
-    while (wpb > 1 && per_wave * wpb + table + kStaticLdsBf16 > kLdsLimit) --wpb;
+    const int wpb = 4;
     const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
-    hipLaunchKernelGGL((k_gram_bf16<R, KIND>), dim3(grid), dim3(64 * wpb),
-                       per_wave * wpb + table, st, a, wpb, O);
+    if (any)
+      hipLaunchKernelGGL((k_gram_bf16<R, KIND>), dim3(grid), dim3(64 * wpb),
+                         per_wave * wpb + table, st, a, wpb, O);
+    return KIND == kBf16 ? GT_DELTA_BF16 : GT_DELTA_BF16R;
   }
 }
 
-template <int KIND> void dispatch(int R, const GramArgs &a, int O, hipStream_t st) {
+template <int KIND> int dispatch(int R, const GramArgs &a, int O, hipStream_t st) {
   switch (R) {
 #define GT_CASE(r)                                                                               \
   case r:                                                                                        \
-    launch_impl<r, KIND>(a, O, st);                                                              \
-    break;
+    return launch_impl<r, KIND>(a, O, st);
     GT_CASE(1) GT_CASE(2) GT_CASE(3) GT_CASE(4) GT_CASE(5) GT_CASE(6) GT_CASE(7) GT_CASE(8)
     GT_CASE(9) GT_CASE(10) GT_CASE(11) GT_CASE(12) GT_CASE(13) GT_CASE(14) GT_CASE(15)
 #undef GT_CASE
@@ -842,14 +859,16 @@ template <int KIND> void dispatch(int R, const GramArgs &a, int O, hipStream_t s
 
 bool gram_mfma_supported(int R) { return R >= 1 && R <= 15; }
 
-void launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s) {
+int launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s) {
+  int path;
   if (precision == GT_PREC_FP32)
-    dispatch<kValu>(R, a, O, s);
+    path = dispatch<kValu>(R, a, O, s);
   else if (precision == GT_PREC_BF16)
-    dispatch<kBf16>(R, a, O, s);
+    path = dispatch<kBf16>(R, a, O, s);
   else
-    dispatch<kBf16R>(R, a, O, s);
+    path = dispatch<kBf16R>(R, a, O, s);
   GT_HIP(hipGetLastError());
+  return path;
 }
 
 void launch_split_factor(int R, const float *A, uint64_t rows, uint32_t *As, hipStream_t s) {

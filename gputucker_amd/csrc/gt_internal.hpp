@@ -35,18 +35,29 @@ __host__ __device__ constexpr int row_stride(int R) {
   return R <= 4 ? 4 : (R <= 8 ? 8 : (R + 15) & ~15);
 }
 
-// Ranks the templated kernels are instantiated for.  A rank outside this list
-// is rejected with GT_ERR_UNSUPPORTED by the C ABI.
+// Ranks the templated kernels are instantiated for.  Any other rank up to
+// GT_MAX_RANK runs at the next one in this list (kernel_rank): the factors'
+// extra columns and the core's extra slices are zero, so the extra delta
+// columns are zero, the extra rows and columns of B are lambda I and of C
+// zero, and the row solves give zero there and leave the real block as it
+// would be at the rank itself.
 #define GT_FOR_EACH_RANK(X) \
   X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) \
-  X(20) X(24) X(32)
+  X(20) X(24) X(28) X(30) X(32) X(36) X(40) X(44) X(50)
 
-inline bool rank_supported(int R) {
-#define GT_RANK_CASE(r) if (R == r) return true;
+// The instantiated rank the kernels run a rank-R model at (0: above GT_MAX_RANK).
+inline int kernel_rank(int R) {
+  int k = 0;
+#define GT_RANK_CASE(r) if (k == 0 && R <= r) k = r;
   GT_FOR_EACH_RANK(GT_RANK_CASE)
 #undef GT_RANK_CASE
-  return false;
+  return R >= 1 ? k : 0;
 }
+
+// LDS bytes one workgroup may allocate on the current device (set at context
+// creation from hipDeviceAttributeMaxSharedMemoryPerBlock; 64 KiB until then).
+size_t lds_limit();
+void set_lds_limit(size_t bytes);
 
 struct HipError : std::runtime_error {
   explicit HipError(const std::string &m) : std::runtime_error(m) {}
@@ -205,7 +216,8 @@ __host__ __device__ constexpr int gb_pairs(int O) { return ((O + 1) / 2 + 1) / 2
 __host__ __device__ constexpr size_t gb_k32_pair_words(int O) {
   return (size_t)gb_pairs(O) * kWave * 8;
 }
-void launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
+// Returns the gt_delta_path it took.
+int launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
 // Split BF16 copy of a factor for the MFMA delta (R <= 15): 16 uint32 words
 // per row, the K=32 operand words of each lane group (bf16r_plan).
 constexpr int kSplitWords = 16;

@@ -25,7 +25,7 @@ void CommandLineOptions::PrintHelp() const {
                "  -h [ --help ]                 Display help menu.\n"
                "  -i [ --input ] arg            Input tensor path\n"
                "  -o [ --order ] arg            Order\n"
-               "  -r [ --rank ] arg (=10)       Rank\n"
+               "  -r [ --rank ] arg (=10)       Rank (1..50; BF16 precisions 1..15)\n"
                "  -g [ --gpus ] arg (=1)        The number of GPUs\n"
                "  --iters arg (=3)              Maximum ALS iterations\n"
                "  --lambda arg (=0.0001)        Ridge regularisation\n"

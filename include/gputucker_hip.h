@@ -28,7 +28,7 @@ extern "C" {
 #endif
 
 #define GT_MAX_ORDER 8 /* constants.hpp:16 kMaxOrder */
-#define GT_MAX_RANK 32
+#define GT_MAX_RANK 50 /* the reference's delta kernel holds R <= 50 sums (delta.cuh:63) */
 
 typedef struct gt_ctx gt_ctx;
 
@@ -47,6 +47,16 @@ typedef enum gt_precision {
   GT_PREC_BF16 = 1,  /* BF16 MFMA contraction, FP32 accumulate */
   GT_PREC_BF16R = 2  /* BF16 MFMA + split-BF16 refinement terms (near-FP32 delta) */
 } gt_precision;
+
+/* The delta + Gram kernel a mode update ran (gt_timings.delta_path). */
+typedef enum gt_delta_path {
+  GT_DELTA_NONE = 0,      /* no mode update has run */
+  GT_DELTA_FP32 = 1,      /* FP32 VALU delta, FP64 MFMA Gram (R <= 15) */
+  GT_DELTA_FP32_WIDE = 2, /* FP32 VALU delta, lane-per-entry FP64 Gram (R > 15) */
+  GT_DELTA_BF16 = 3,      /* BF16 MFMA delta, core table in LDS */
+  GT_DELTA_BF16R = 4,     /* refined BF16 MFMA delta, core table in LDS */
+  GT_DELTA_BF16R_L2 = 5   /* refined BF16 MFMA delta, core operand from L2 (orders 4-5) */
+} gt_delta_path;
 
 typedef struct gt_opts {
   int device;                       /* HIP device ordinal */
@@ -102,6 +112,8 @@ typedef struct gt_timings {
   double ms_partition;            /* partition + row grouping */
   double gram_kernel_ms_total;    /* with opts.profile: summed event time of the Gram kernel */
   uint64_t gram_kernel_launches;
+  int delta_path[GT_MAX_ORDER];   /* gt_delta_path of the last update of each mode: a BF16
+                                     request that no MFMA kernel covers runs GT_DELTA_FP32 */
 } gt_timings;
 
 /* Defaults as the reference compiles them in (constants.hpp:16-18,

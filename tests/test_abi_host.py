@@ -95,7 +95,7 @@ def test_create_without_gpu_fails_cleanly():
 
 def test_create_rejects_bad_options():
     with pytest.raises(gt.GtError) as e:
-        gt.Engine(3, 17)  # rank not built
+        gt.Engine(3, 51)  # above the reference's 50 (delta.cuh:63)
     assert e.value.code == 4
     with pytest.raises(gt.GtError) as e:
         gt.Engine(1, 4)  # order 1
