@@ -11,6 +11,11 @@ of (iteration time without the fit) / N, and value = nnz / t_mode.  The fit
 is computed after the timed region.
 
     python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5s] [--precision fp32]
+                    [--no-graph] [--no-autotune] [--budget BYTES] [--no-cpu]
+
+By default each ALS iteration replays a captured HIP graph (the N mode
+updates and the fit reduction) and the chunk size of each mode is picked by
+the engine's autotuner at initialisation.
 """
 from __future__ import annotations
 
@@ -57,51 +62,88 @@ CONFIGS = {
 
 HBM_PEAK_GBS = 8000.0
 
+# the delta + Gram kernel behind each gt_delta_path, as rocprofv3 names it
+_PATH_KERNEL = {"fp32": "k_gram_valu<{R}>", "fp32-wide": "k_gram_fp32<{R}>",
+                "bf16": "k_gram_bf16<{R}, 1>", "bf16r": "k_gram_bf16<{R}, 2>",
+                "bf16r-l2": "k_gram_bf16_o4<{R}, {NS}>"}
+# the arithmetic each path's delta runs in
+_PATH_DTYPE = {"fp32": "fp32", "fp32-wide": "fp32", "bf16": "bf16", "bf16r": "bf16r",
+               "bf16r-l2": "bf16r"}
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on `workload` from the committed
-    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, scaled by the
-    factors the same counters read for a kernel of known traffic), or None."""
+
+def committed_counters(workload, kernel):
+    """The committed rocprofv3 counter summary of `kernel` on `workload`
+    (profiles/counters_<workload>.json, written from FETCH_SIZE / WRITE_SIZE
+    runs scaled by a copy kernel's factors and from SQ_ counter runs), or None
+    when it does not describe this kernel."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"traffic_{workload}.json")
+                        f"counters_{workload}.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kernel or not t.get("traffic_bytes_per_launch"):
-        return None
-    return t["traffic_bytes_per_launch"]
+    k = t.get("kernels", {}).get(kernel)
+    return k if k else None
 
 
-def kernel_name(precision, R):
-    """The delta + Gram kernel a precision runs at rank R."""
-    if R > 15:
-        return f"k_gram_fp32<{R}>"
-    kind = {"fp32": None, "bf16": 1, "bf16r": 2}[precision]
-    return f"k_gram_valu<{R}>" if kind is None else f"k_gram_bf16<{R}, {kind}>"
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary, or None."""
+    k = committed_counters(workload, kernel)
+    return k.get("traffic_bytes_per_launch") if k else None
+
+
+def kernel_name(path, R, N=3):
+    """The delta + Gram kernel of a gt_delta_path at kernel rank R."""
+    return _PATH_KERNEL[path].format(R=R, NS=max(N - 2, 0))
 
 
 def bytes_per_nnz(N, R, s_f):
     """Algorithmic HBM bytes per nonzero per mode update: N indices, the value,
-    the row-grouping entry and the N-1 gathered factor rows."""
+    the row-grouping entry and the N-1 gathered factor rows of s_f bytes per
+    element (4 FP32, 2 BF16)."""
     return 4 * N + 8 + (N - 1) * R * s_f
+
+
+def kernel_bytes_per_nnz(N, R, path):
+    """Bytes per nonzero the delta + Gram kernel itself requests: N stream
+    indices and the value (no row-grouping entry: the stream is row-sorted),
+    the split BF16 copy of the last other factor (16 words per row) on the
+    MFMA paths or its FP32 row, and the other factors' FP32 rows."""
+    a_l = 64 if path.startswith("bf16") else 4 * R
+    return 4 * N + 4 + a_l + (N - 2) * 4 * R
 
 
 def flops_per_nnz(N, R):
     return 2 * R ** N + 2 * R * R + 2 * R
 
 
-def cpu_baseline(idx, vals, dims, part, R, lam, A, G, target_s=20.0):
-    """Times the oracle's OpenMP restatement of the reference host path (delta
-    as the reference kernel computes it, then B/C and the row solves) for one
-    mode update on two samples of the workload's nonzeros (the first S and 2S
-    in input order, full index space), and fits t = rows_cost + nnz * per_nnz
-    to extrapolate to the full nonzero count."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(idx, vals, dims, part, R, lam, A, G, sample=20_000_000):
+    """Times the oracle's restatement of the reference host path, built with
+    -O3 -march=native and OpenMP (libbaseline.so): for every mode in turn, the
+    delta as the reference kernel computes it (delta.cuh:62-90, parallel over
+    nonzeros), B/C (update.cuh:35-101) and the LU row solves
+    (update.cuh:167-194), in FP32 as the reference's FP32 build.  It runs one
+    ALS iteration's N mode updates on the first `sample` nonzeros of the
+    workload (input order, full index space and partition) and reports
+    sample / (time / N): measured, not extrapolated."""
     import oracle
     N = len(dims)
-
-    def t_mode(S):
+    S = min(int(sample), idx.shape[1])
+    oracle.use_baseline(True)
+    try:
         t = oracle.Tensor(idx[:, :S], vals[:S].astype(np.float32), dims, part, R)
         Ap = [oracle.pad_factor(a, t.rows(k)) for k, a in enumerate(A)]
         t0 = time.perf_counter()
@@ -109,62 +151,75 @@ def cpu_baseline(idx, vals, dims, part, R, lam, A, G, target_s=20.0):
             d = oracle.delta(t, Ap, G, m, "f32")
             B, Cv = oracle.bc(t, d, m, lam, "f32")
             Ap[m] = oracle.solve(B, Cv, "f32")
-        return (time.perf_counter() - t0) / N
-
-    S = 250_000
-    t1 = t_mode(S)
-    while t1 < target_s / 8 and 4 * S <= idx.shape[1]:
-        S *= 2
-        t1 = t_mode(S)
-    t2 = t_mode(2 * S)
-    per_nnz = max((t2 - t1) / S, 1e-12)
-    rows_cost = max(t1 - S * per_nnz, 0.0)
-    t_full = rows_cost + idx.shape[1] * per_nnz
-    return dict(value=idx.shape[1] / t_full, unit="nnz/s", cores=os.cpu_count(), kind="port",
-                sample=(f"oracle FP32 OpenMP mode update on the first {S} and {2 * S} nonzeros "
-                        f"(full index space), extrapolated linearly to {idx.shape[1]} nnz: "
-                        f"per-nnz {per_nnz * 1e9:.1f} ns, per-mode row cost {rows_cost:.2f} s"))
+            del d, B, Cv
+        t_mode = (time.perf_counter() - t0) / N
+        threads = oracle.max_threads()
+    finally:
+        oracle.use_baseline(False)
+    return dict(value=S / t_mode, unit="nnz/s", cores=threads, kind="port",
+                cpu=cpu_model(), ms_per_mode=t_mode * 1e3,
+                sample=(f"one ALS iteration ({N} mode updates: delta, B/C, LU row solves, FP32) "
+                        f"of the oracle built -O3 -march=native on the first {S} of the "
+                        f"{idx.shape[1]} nonzeros (full index space, partition {list(part)}), "
+                        f"{threads} OpenMP threads; value = {S} / (time / {N})"))
 
 
 def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup, chunk_tiles,
-               budget=0):
-    """Partitions, initialises and times `steps` mode updates; returns a dict."""
+               budget=0, graph=False, autotune=False):
+    """Partitions, initialises, runs `warmup` untimed mode updates, then times
+    exactly `steps` mode updates (whole ALS iterations of N updates, then
+    single updates for the remainder) between device synchronisations.  With
+    graph, the iterations are replays of a captured HIP graph, which holds no
+    timing events: the delta + Gram kernel's time then comes from a second,
+    eager engine that runs the same steps."""
     N = len(dims)
+    if graph:
+        r = run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup,
+                       chunk_tiles, budget, False, autotune)
     e = gt.Engine(N, R, partition=part, lam=lam, precision=precision, chunk_tiles=chunk_tiles,
-                  profile=True, device_budget=budget)
+                  profile=not graph, device_budget=budget, graph=graph, autotune=autotune)
     t0 = time.perf_counter()
     e.set_coo(idx, vals, dims)
     e.partition()
     e.set_init(A, G)
     e.synchronize()
     setup_s = time.perf_counter() - t0
-    for s in range(max(warmup // N, 1)):
-        e.iteration()
+
+    def updates(k):
+        its = []
+        for _ in range(k // N):
+            ti = time.perf_counter()
+            e.iteration()  # N mode updates and the fit from the row solves
+            its.append(time.perf_counter() - ti)
+        for m in range(k % N):
+            e.mode_update(m)
+        return its
+
+    updates(warmup)
     e.synchronize()
     base = e.timings()
-    it_times = []
     t_start = time.perf_counter()
-    fit = None
-    for it in range(max(steps // N, 1)):  # whole ALS iterations: N mode updates + the fit
-        ti = time.perf_counter()
-        fit = e.iteration()
-        it_times.append(time.perf_counter() - ti)
+    it_times = updates(steps)
+    e.synchronize()
     total = time.perf_counter() - t_start
     tm = e.timings()
-    t_fit = time.perf_counter()
-    fit_recon = e.fit()  # the reconstruction pass of reconstruction.cuh, timed once
-    t_fit = time.perf_counter() - t_fit
+    fit = e.fit()  # the reconstruction pass (reconstruction.cuh), after the timed region
     launches = tm["gram_kernel_launches"] - base["gram_kernel_launches"]
     k_ms = (tm["gram_kernel_ms_total"] - base["gram_kernel_ms_total"]) / max(launches, 1)
-    t_mode = statistics.median(it_times) / N
+    t_mode = statistics.median(it_times) / N if it_times else total / steps
     info = e.info()
     e.close()
-    return dict(t_mode=t_mode, value=idx.shape[1] / t_mode, fit=fit, fit_recon=fit_recon,
+    if graph:  # the kernel times and delta path of the eager engine
+        launches, k_ms = r["launches"], r["kernel_ms"]
+        tm = dict(tm, ms_gram=[r["gram_phase_ms"]], ms_solve=[r["solve_ms"]],
+                  delta_path=r["delta_path"])
+    return dict(t_mode=t_mode, value=idx.shape[1] / t_mode, fit=fit,
                 out_of_core=info["out_of_core"], segment_nnz=info["segment_nnz"],
                 kernel_ms=k_ms, gram_phase_ms=statistics.mean(tm["ms_gram"]),
-                solve_ms=statistics.mean(tm["ms_solve"]),
-                launches=launches, total=total, setup_s=setup_s, fit_s=t_fit,
-                partition_ms=tm["ms_partition"], part=[int(p) for p in info["part"]])
+                solve_ms=statistics.mean(tm["ms_solve"]), delta_path=tm["delta_path"],
+                launches=launches, total=total, setup_s=setup_s,
+                partition_ms=tm["ms_partition"], part=[int(p) for p in info["part"]],
+                tuned_chunk_tiles=tm["tuned_chunk_tiles"], graph=graph)
 
 
 def main(argv=None):
@@ -181,6 +236,11 @@ def main(argv=None):
                     help="device budget in bytes; below the resident size the streams are "
                          "staged from pinned host memory every mode update (PCIe-inclusive)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="launch each iteration's kernels eagerly instead of replaying the "
+                         "captured HIP graph")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default chunk size instead of timing the candidates")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is "
@@ -192,10 +252,14 @@ def main(argv=None):
         # the BF16 delta is built for R <= 15; above that FP32 is the only path
         print(f"[bench] R = {R}: timing the FP32 path only", file=sys.stderr, flush=True)
         a.precision, a.compare = "fp32", ""
+    if a.compare == a.precision:
+        a.compare = ""
 
     steps = a.steps if a.steps is not None else cfg["iters"] * N
     warmup = a.warmup if a.warmup is not None else N
-    steps = max(N, (steps // N) * N)
+    if steps < 1 or warmup < 0:
+        print(json.dumps({"error": "--steps must be >= 1 and --warmup >= 0"}))
+        return 2
 
     import gputucker_amd as gt
     from gputucker_amd import synth
@@ -213,18 +277,23 @@ def main(argv=None):
     runs = {}
     for prec in [p for p in (a.compare, a.precision) if p]:
         r = run_engine(gt, idx, vals, dims, A, G, cfg["part"], R, lam, prec, steps, warmup,
-                       a.chunk_tiles, a.budget)
+                       a.chunk_tiles, a.budget, a.graph, a.autotune)
         runs[prec] = r
         print(f"[bench] {prec}: t_mode {r['t_mode'] * 1e3:.3f} ms ({r['value']:.3e} nnz/s), "
-              f"gram kernel {r['kernel_ms']:.3f} ms avg over {r['launches']} "
-              f"(gram phase {r['gram_phase_ms']:.3f} ms, solve {r['solve_ms']:.3f} ms), "
-              f"out of core {r['out_of_core']} (segment {r['segment_nnz']}), fit {r['fit']:.7f} "
-              f"(reconstruction {r['fit_recon']:.7f}), setup {r['setup_s']:.1f}s "
-              f"(partition {r['partition_ms']:.0f} ms, P {r['part']}), reconstruction pass "
-              f"{r['fit_s'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+              f"delta path {r['delta_path']}, gram kernel {r['kernel_ms']:.3f} ms avg over "
+              f"{r['launches']} (gram phase {r['gram_phase_ms']:.3f} ms, solve "
+              f"{r['solve_ms']:.3f} ms), out of core {r['out_of_core']} (segment "
+              f"{r['segment_nnz']}), fit {r['fit']:.7f}, setup {r['setup_s']:.1f}s (partition "
+              f"{r['partition_ms']:.0f} ms, P {r['part']})", file=sys.stderr, flush=True)
     head = runs[a.precision]
-    bpn = bytes_per_nnz(N, R, 4)  # factor rows are gathered as FP32 on every path
+    paths = sorted(set(head["delta_path"]))
+    dtype = "+".join(sorted({_PATH_DTYPE[p] for p in paths}))
+    RK = gt.kernel_rank(R)
+    kernel = kernel_name(paths[0], RK, N) if len(paths) == 1 else None
+    s_f = 2 if dtype.startswith("bf16") and len(paths) == 1 else 4
+    bpn = bytes_per_nnz(N, R, s_f)
     achieved = bpn * idx.shape[1] / (head["kernel_ms"] * 1e-3) / 1e9
+    ctr = committed_counters(a.config, kernel) if kernel else None
 
     cpu = None
     if not a.no_cpu:
@@ -240,25 +309,33 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": a.precision,
+        "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": cfg["name"], "nnz": int(idx.shape[1]), "order": N, "rank": R,
                    "partition": head["part"], "precision": a.precision,
+                   "delta_path": head["delta_path"], "graph": a.graph,
+                   "chunk_tiles": head["tuned_chunk_tiles"] if a.autotune else a.chunk_tiles,
                    "parallelism": "single GPU",
                    "out_of_core": bool(head["out_of_core"])},
         "fit": head["fit"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(a.config, kernel_name(a.precision, R)),
-                     "kernel": kernel_name(a.precision, R), "kernel_ms": head["kernel_ms"],
-                     "bytes_per_nnz": bpn},
+                     "traffic": ctr.get("traffic_bytes_per_launch") if ctr else None,
+                     "kernel": kernel, "kernel_ms": head["kernel_ms"],
+                     "bytes_per_nnz": bpn,
+                     "kernel_bytes_per_nnz": kernel_bytes_per_nnz(N, RK, paths[0]),
+                     "flops_per_nnz": flops_per_nnz(N, R),
+                     "mfma_util": ctr.get("mfma_busy") if ctr else None,
+                     "valu_util": ctr.get("valu_busy") if ctr else None,
+                     "hbm_util": ctr.get("hbm_util") if ctr else None,
+                     "limiter": ctr.get("limiter") if ctr else None},
         "cpu_baseline": cpu,
     }
     if a.compare and a.compare in runs:
         base = runs[a.compare]
         out["compare"] = {"precision": a.compare, "value": base["value"],
                           "ms_per_step": base["t_mode"] * 1e3, "fit": base["fit"],
-                          "kernel_ms": base["kernel_ms"],
+                          "delta_path": base["delta_path"], "kernel_ms": base["kernel_ms"],
                           "speedup": head["value"] / base["value"],
                           "fit_rel_diff": abs(head["fit"] - base["fit"]) / abs(base["fit"])}
     print(json.dumps(out))

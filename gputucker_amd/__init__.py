@@ -33,7 +33,7 @@ EXPORTS = [
     "gt_iteration", "gt_run",
     "gt_get_factor", "gt_set_factor", "gt_get_delta", "gt_get_stream_delta", "gt_get_gram",
     "gt_get_recon",
-    "gt_get_timings", "gt_synchronize", "gt_version",
+    "gt_get_timings", "gt_synchronize", "gt_version", "gt_kernel_rank", "gt_autotune",
 ]
 
 
@@ -43,6 +43,7 @@ class GtOpts(C.Structure):
         ("max_iters", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
         ("partition", C.c_uint32 * MAX_ORDER), ("device_budget", C.c_uint64),
         ("chunk_tiles", C.c_int), ("profile", C.c_int), ("ref_fit", C.c_int),
+        ("autotune", C.c_int), ("graph", C.c_int),
     ]
 
 
@@ -63,7 +64,10 @@ class GtTimings(C.Structure):
         ("ms_gram", C.c_double * MAX_ORDER), ("ms_solve", C.c_double * MAX_ORDER),
         ("ms_mode", C.c_double * MAX_ORDER), ("ms_fit", C.c_double),
         ("ms_partition", C.c_double), ("gram_kernel_ms_total", C.c_double),
-        ("gram_kernel_launches", C.c_uint64), ("delta_path", C.c_int * MAX_ORDER),
+        ("gram_kernel_launches", C.c_uint64), ("ms_iteration_graph", C.c_double),
+        ("tuned_chunk_tiles", C.c_int * MAX_ORDER), ("tuned_variant", C.c_int * MAX_ORDER),
+        ("tuned_ms", C.c_double * MAX_ORDER),
+        ("delta_path", C.c_int * MAX_ORDER),
     ]
 
 
@@ -115,12 +119,19 @@ def lib():
         L.gt_get_recon.argtypes = [vp, vp]
         L.gt_get_timings.argtypes = [vp, C.POINTER(GtTimings)]
         L.gt_synchronize.argtypes = [vp]
+        L.gt_kernel_rank.argtypes = [C.c_int]
+        L.gt_autotune.argtypes = [vp]
         _lib = L
     return _lib
 
 
 def _u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def kernel_rank(rank):
+    """The rank the kernels run a rank-`rank` model at (zero-padded above it)."""
+    return lib().gt_kernel_rank(int(rank))
 
 
 def default_lambda():
@@ -164,7 +175,8 @@ class Engine:
     """One device context.  Mirrors the reference's TuckerDecomposition flow."""
 
     def __init__(self, order, rank, device=0, lam=None, max_iters=3, tol=None, precision="fp32",
-                 partition=None, device_budget=0, chunk_tiles=0, profile=False, ref_fit=False):
+                 partition=None, device_budget=0, chunk_tiles=0, profile=False, ref_fit=False,
+                 graph=False, autotune=False):
         L = lib()
         o = GtOpts()
         L.gt_default_opts(C.byref(o))
@@ -182,6 +194,8 @@ class Engine:
         o.chunk_tiles = int(chunk_tiles)
         o.profile = 1 if profile else 0
         o.ref_fit = 1 if ref_fit else 0
+        o.graph = 1 if graph else 0
+        o.autotune = 1 if autotune else 0
         self.opts = o
         self.order, self.rank = int(order), int(rank)
         self._h = C.c_void_p()
@@ -265,6 +279,13 @@ class Engine:
         ptrs = (C.c_void_p * self.order)(*[a.ctypes.data for a in A])
         self._check(lib().gt_set_init(self._h, ptrs, core.ctypes.data))
 
+    def autotune(self):
+        """Per-mode chunk size (and core-operand source) by timing the pass."""
+        self._check(lib().gt_autotune(self._h))
+        t = self.timings()
+        return dict(chunk_tiles=t["tuned_chunk_tiles"], variant=t["tuned_variant"],
+                    ms=t["tuned_ms"])
+
     def mode_update(self, mode):
         self._check(lib().gt_mode_update(self._h, int(mode)))
 
@@ -332,6 +353,9 @@ class Engine:
                     ms_mode=list(t.ms_mode[:N]), ms_fit=t.ms_fit, ms_partition=t.ms_partition,
                     gram_kernel_ms_total=t.gram_kernel_ms_total,
                     gram_kernel_launches=t.gram_kernel_launches,
+                    ms_iteration_graph=t.ms_iteration_graph,
+                    tuned_chunk_tiles=list(t.tuned_chunk_tiles[:N]),
+                    tuned_variant=list(t.tuned_variant[:N]), tuned_ms=list(t.tuned_ms[:N]),
                     delta_path=[DELTA_PATHS[p] for p in t.delta_path[:N]])
 
     def synchronize(self):

@@ -229,6 +229,10 @@ struct gt_ctx {
   DevBuf<float> xhat;          // opts.ref_fit: the reconstruction of every nonzero, block order
   float *dbg_delta = nullptr;  // test hook: the Gram kernels' deltas go here (stream order)
   uint64_t chunk = 0, nchunks = 0;
+  // per-mode chunk size and delta variant picked by the autotuner (in core;
+  // 0 = the default chunk, the default variant)
+  uint64_t mchunk[GT_MAX_ORDER] = {};
+  int mvariant[GT_MAX_ORDER] = {};
   // out-of-core pipeline: the per-mode streams live in pinned host memory and
   // are pumped through two device staging buffers on a copy stream
   bool streaming = false;
@@ -250,6 +254,15 @@ struct gt_ctx {
   hipEvent_t ev_copy[2] = {}, ev_comp[2] = {};
   // timing
   double *scratch = nullptr; // this device's address of the static scratch array
+  // opts.graph: one in-core ALS iteration (N mode updates and the reduction
+  // of the fit partials) captured once, replayed by later iterations
+  bool capturing = false;
+  hipGraphExec_t it_graph = nullptr;
+  uint64_t it_graph_parts = 0;
+  void drop_graph() {
+    if (it_graph) (void)hipGraphExecDestroy(it_graph);
+    it_graph = nullptr;
+  }
   hipEvent_t ev[8] = {};
   hipEvent_t ev_mode[GT_MAX_ORDER][3] = {}; // per mode: start, Gram done, solve done
   bool mode_pending[GT_MAX_ORDER] = {};     // recorded, not yet read into tm
@@ -318,6 +331,7 @@ struct gt_ctx {
   ~gt_ctx() {
     if (s) {
       (void)hipStreamSynchronize(s);
+      drop_graph();
       for (auto &pr : prof_pending) {
         (void)hipEventDestroy(pr.first);
         (void)hipEventDestroy(pr.second);
@@ -614,10 +628,11 @@ GramArgs gram_args(gt_ctx *c, int n) {
   a.bpart0 = c->bpart0.p;
   a.bpart1 = c->bpart1.p;
   a.chunk_tail = c->chunk_tail.p;
-  a.chunk = c->chunk;
-  a.nchunks = c->nchunks;
+  a.chunk = (c->mchunk[n] && !c->streaming) ? c->mchunk[n] : c->chunk;
+  a.nchunks = (c->nnz + a.chunk - 1) / a.chunk;
   a.chunk_begin = 0;
-  a.chunk_end = c->nchunks;
+  a.chunk_end = a.nchunks;
+  a.prefer_l2 = c->streaming ? 0 : c->mvariant[n];
   a.rows = c->rows[n];
   a.Gb = c->Gb[n].p;
   // slot order: L = last mode other than n, outer slots = the rest ascending
@@ -654,7 +669,8 @@ void set_stream_ptrs(GramArgs &a, const gt_ctx *c, int n, uint32_t *const *idx,
 
 void launch_gram_window(gt_ctx *c, const GramArgs &a) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (c->o.profile) {
+  const bool prof = c->o.profile && !c->capturing;
+  if (prof) {
     e0 = c->get_event();
     e1 = c->get_event();
     GT_HIP(hipEventRecord(e0, c->s));
@@ -667,7 +683,7 @@ void launch_gram_window(gt_ctx *c, const GramArgs &a) {
     path = GT_DELTA_FP32_WIDE;
   }
   c->tm.delta_path[a.mode] = path;
-  if (c->o.profile) {
+  if (prof) {
     GT_HIP(hipEventRecord(e1, c->s));
     c->prof_pending.emplace_back(e0, e1);
   }
@@ -728,6 +744,76 @@ void run_gram(gt_ctx *c, int n) {
   launch_gram_fixup(c->R, a, c->s);
 }
 
+// The project's autotuner (in core): for each mode, times the delta + Gram
+// pass (with its fix-up) at chunk sizes of 1 to 32 tiles and, where both
+// kernels apply (refined BF16 at orders 4-5 with a core table that fits in
+// LDS), with the core operand from LDS and from L2; keeps the fastest.  The
+// passes only overwrite the Gram scratch, not the factors.  The chunking
+// changes the order of the FP64 sums inside a row, so results move at FP64
+// rounding.
+void autotune(gt_ctx *c) {
+  require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
+  if (c->streaming) return; // the staged segments fix the chunk size
+  c->drop_graph();
+  const int W = gram_width(c->R);
+  const uint64_t max_chunks = (c->nnz + kTile - 1) / kTile;
+  c->bpart0.alloc(std::max<uint64_t>(max_chunks, 1) * W);
+  c->bpart1.alloc(std::max<uint64_t>(max_chunks, 1) * W);
+  c->chunk_tail.alloc(std::max<uint64_t>(max_chunks, 1));
+  const bool prof = c->o.profile != 0;
+  c->o.profile = 0;
+  hipEvent_t e0 = c->get_event(), e1 = c->get_event();
+  for (int n = 0; n < c->N; ++n) {
+    double best = 1e30;
+    uint64_t best_chunk = c->chunk;
+    int best_var = 0;
+    for (int var = 0; var < 2; ++var) {
+      if (var == 1) { // the L2 variant: only where it differs from the default
+        c->mvariant[n] = 1;
+        GramArgs probe = gram_args(c, n);
+        probe.chunk_end = 0; // pick the kernel, launch nothing
+        const int p1 = gram_mfma_supported(c->R) ? launch_gram_mfma(c->R, c->o.precision, probe,
+                                                                    c->O, c->s) : 0;
+        c->mvariant[n] = 0;
+        probe = gram_args(c, n);
+        probe.chunk_end = 0;
+        const int p0 = gram_mfma_supported(c->R) ? launch_gram_mfma(c->R, c->o.precision, probe,
+                                                                    c->O, c->s) : 0;
+        if (p1 == p0) continue;
+      }
+      for (uint64_t tiles = 1; tiles <= 32; tiles *= 2) {
+        const uint64_t chunk = tiles * kTile;
+        if (tiles > 1 && (c->nnz + chunk - 1) / chunk < 256) break; // too few waves
+        c->mchunk[n] = chunk;
+        c->mvariant[n] = var;
+        float ms_best = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) { // the first run warms the caches
+          GT_HIP(hipEventRecord(e0, c->s));
+          run_gram(c, n);
+          GT_HIP(hipEventRecord(e1, c->s));
+          GT_HIP(hipEventSynchronize(e1));
+          float ms = 0.f;
+          GT_HIP(hipEventElapsedTime(&ms, e0, e1));
+          if (rep > 0) ms_best = std::min(ms_best, ms);
+        }
+        if (ms_best < best) {
+          best = ms_best;
+          best_chunk = chunk;
+          best_var = var;
+        }
+      }
+    }
+    c->mchunk[n] = best_chunk;
+    c->mvariant[n] = best_var;
+    c->tm.tuned_chunk_tiles[n] = (int)(best_chunk / kTile);
+    c->tm.tuned_variant[n] = best_var;
+    c->tm.tuned_ms[n] = best;
+  }
+  c->ev_pool.push_back(e0);
+  c->ev_pool.push_back(e1);
+  c->o.profile = prof;
+}
+
 // Refreshes the split BF16 copy of factor n that the MFMA delta gathers
 // (BF16 precisions only).
 void refresh_split(gt_ctx *c, int n) {
@@ -744,9 +830,10 @@ uint64_t mode_update(gt_ctx *c, int n, bool fit_partials, bool sync) {
   require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
   require(n >= 0 && n < c->N, GT_ERR_ARG, "mode out of range");
   if (c->prof_pending.size() > 64) c->drain_profile();
-  GT_HIP(hipEventRecord(c->ev_mode[n][0], c->s));
+  const bool timed = !c->capturing; // no event nodes in a captured iteration
+  if (timed) GT_HIP(hipEventRecord(c->ev_mode[n][0], c->s));
   run_gram(c, n);
-  GT_HIP(hipEventRecord(c->ev_mode[n][1], c->s));
+  if (timed) GT_HIP(hipEventRecord(c->ev_mode[n][1], c->s));
   SolveArgs sa{};
   sa.gram = c->gram.p;
   sa.row_ptr = c->row_ptr[n].p;
@@ -756,6 +843,7 @@ uint64_t mode_update(gt_ctx *c, int n, bool fit_partials, bool sync) {
   sa.fit_partials = fit_partials ? c->fit_part.p : nullptr;
   sa.As = c->As[n].p; // the solve refreshes the split copy (BF16 precisions)
   const uint64_t parts = launch_solve(c->R, sa, c->s);
+  if (!timed) return parts;
   GT_HIP(hipEventRecord(c->ev_mode[n][2], c->s));
   c->mode_pending[n] = true;
   if (sync) {
@@ -913,11 +1001,46 @@ double iteration(gt_ctx *c) {
     for (int n = 0; n < c->N; ++n) mode_update(c, n, false, false);
     return compute_fit_ref(c);
   }
-  // the N updates are queued back to back; the host waits once, for the fit
-  for (int n = 0; n + 1 < c->N; ++n) mode_update(c, n, false, false);
-  const uint64_t parts = mode_update(c, c->N - 1, true, false);
-  GT_HIP(hipEventRecord(c->ev[3], c->s));
-  reduce_f64(c->fit_part.p, (int)parts, c->scalar.p, c->s);
+  if (c->o.graph && !c->streaming) {
+    // the N updates and the reduction of the fit partials as one graph:
+    // kernels only (no event, copy or memset nodes), captured on first use
+    // and replayed; every buffer it names stays put until the graph is
+    // dropped (gt_set_coo, gt_partition, gt_set_init)
+    if (!c->it_graph) {
+      require(c->partitioned && c->initialized, GT_ERR_STATE, "partition and init first");
+      hipGraph_t g = nullptr;
+      c->drain_profile(); // no event waits while the stream is captured
+      GT_HIP(hipStreamSynchronize(c->s));
+      GT_HIP(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+      c->capturing = true;
+      uint64_t parts = 0;
+      try {
+        for (int n = 0; n + 1 < c->N; ++n) mode_update(c, n, false, false);
+        parts = mode_update(c, c->N - 1, true, false);
+        reduce_f64(c->fit_part.p, (int)parts, c->scalar.p, c->s);
+      } catch (...) {
+        c->capturing = false;
+        (void)hipStreamEndCapture(c->s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      c->capturing = false;
+      GT_HIP(hipStreamEndCapture(c->s, &g));
+      const hipError_t e = hipGraphInstantiate(&c->it_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      GT_HIP(e);
+      c->it_graph_parts = parts;
+    }
+    GT_HIP(hipEventRecord(c->ev[5], c->s));
+    GT_HIP(hipGraphLaunch(c->it_graph, c->s));
+    GT_HIP(hipEventRecord(c->ev[3], c->s));
+  } else {
+    // the N updates are queued back to back; the host waits once, for the fit
+    for (int n = 0; n + 1 < c->N; ++n) mode_update(c, n, false, false);
+    const uint64_t parts = mode_update(c, c->N - 1, true, false);
+    GT_HIP(hipEventRecord(c->ev[3], c->s));
+    reduce_f64(c->fit_part.p, (int)parts, c->scalar.p, c->s);
+  }
   double d = 0.0;
   GT_HIP(hipMemcpyAsync(&d, c->scalar.p, sizeof(double), hipMemcpyDeviceToHost, c->s));
   GT_HIP(hipEventRecord(c->ev[4], c->s));
@@ -926,6 +1049,10 @@ double iteration(gt_ctx *c) {
   float ms = 0.f;
   GT_HIP(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
   c->tm.ms_fit = ms;
+  if (c->o.graph && !c->streaming) {
+    GT_HIP(hipEventElapsedTime(&ms, c->ev[5], c->ev[3]));
+    c->tm.ms_iteration_graph = ms;
+  }
   if (c->norm == 0.0) return 1.0; // reconstruction.cuh:272-273
   const double err = std::max(0.0, c->norm * c->norm + d);
   return 1.0 - std::sqrt(err) / c->norm;
@@ -1078,7 +1205,9 @@ void do_partition(gt_ctx *c) {
 // ================================================================ C ABI
 extern "C" {
 
-int gt_version(void) { return 100; }
+int gt_version(void) { return 200; }
+
+int gt_kernel_rank(int rank) { return rank > GT_MAX_RANK ? 0 : kernel_rank(rank); }
 
 int gt_default_opts(gt_opts *o) {
   if (!o) return GT_ERR_ARG;
@@ -1244,6 +1373,7 @@ int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const floa
     for (int k = 0; k < ctx->N; ++k) require(dims[k] > 0, GT_ERR_ARG, "zero dimension");
     for (int k = 0; k < ctx->N; ++k) require(idx[k] != nullptr || nnz == 0, GT_ERR_ARG, "null idx");
     ctx->release_host_streams();
+    ctx->drop_graph();
     ctx->partitioned = false;
     ctx->initialized = false;
     ctx->nnz = nnz;
@@ -1274,6 +1404,7 @@ int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const floa
 int gt_partition(gt_ctx *ctx, uint32_t *part_out, uint64_t *block_nnz_out) {
   return guarded(ctx, [&] {
     require(ctx->nnz > 0 || !ctx->have_coo, GT_ERR_ARG, "empty tensor");
+    ctx->drop_graph();
     do_partition(ctx);
     if (part_out)
       for (int k = 0; k < ctx->N; ++k) part_out[k] = ctx->plan.part[k];
@@ -1386,6 +1517,8 @@ int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core) {
   return guarded(ctx, [&] {
     require(ctx->partitioned, GT_ERR_STATE, "partition first");
     require(A && core, GT_ERR_ARG, "null argument");
+    GT_HIP(hipStreamSynchronize(ctx->s));
+    ctx->drop_graph();
     const int N = ctx->N, R = ctx->R;
     const std::vector<float> core_k = pad_core(core, N, ctx->Ru, R);
     for (int n = 0; n < N; ++n) {
@@ -1407,7 +1540,16 @@ int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core) {
       GT_HIP(hipStreamSynchronize(ctx->s));
     }
     ctx->initialized = true;
+    for (int n = 0; n < GT_MAX_ORDER; ++n) {
+      ctx->mchunk[n] = 0;
+      ctx->mvariant[n] = 0;
+    }
+    if (ctx->o.autotune) autotune(ctx);
   });
+}
+
+int gt_autotune(gt_ctx *ctx) {
+  return guarded(ctx, [&] { autotune(ctx); });
 }
 
 int gt_mode_update(gt_ctx *ctx, int mode) {

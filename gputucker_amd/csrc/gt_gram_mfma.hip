@@ -807,7 +807,8 @@ template <int R, int KIND> int launch_impl(const GramArgs &a, int O, hipStream_t
     const size_t wg4 = per_wave * 4 + table + kStaticLdsBf16;
     const bool table_fits =
         wg4 <= lds_limit() && (int)(lds_limit() / wg4) * 4 >= kMinLdsWavesPerCu;
-    if (!table_fits || a.As_L == nullptr) { // the core operand from L2 instead
+    if (!table_fits || a.As_L == nullptr || (KIND == kBf16R && a.prefer_l2)) {
+      // the core operand from L2 instead
       if constexpr (KIND == kBf16R && bf16r_single_mfma(R) && R >= 2) {
         // orders 4 and 5: the core operand from L2, the outer rows in LDS
         const int wpb = 4;

@@ -170,6 +170,8 @@ struct GramArgs {
   const float *A_O[kMaxOrder - 2];
   const uint32_t *As_L;                  // split BF16 copy of A_L (16 words per row) or null
   float *delta_out;                      // test hook: null, or nnz x R deltas in stream order
+  int prefer_l2;                         // 1: refined BF16 takes the core operand from L2 even
+                                         // where its table fits in LDS (the autotuner's choice)
 };
 void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s);
 void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);

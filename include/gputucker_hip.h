@@ -81,6 +81,12 @@ typedef struct gt_opts {
                                        (reconstruction.cuh:259-276).  Needs the block COO
                                        resident (in core); gt_iteration then runs the
                                        reconstruction pass instead of the fused fit */
+  int autotune;                     /* 1: gt_set_init runs gt_autotune (in core) */
+  int graph;                        /* 1: gt_iteration / gt_run capture one in-core iteration
+                                       (N mode updates and the fit reduction, kernels only)
+                                       into a HIP graph on first use and replay it; dropped
+                                       by gt_set_coo, gt_partition and gt_set_init.  No per-mode
+                                       or per-kernel timings inside it (ms_iteration_graph) */
 } gt_opts;
 
 typedef struct gt_partition_info {
@@ -112,6 +118,10 @@ typedef struct gt_timings {
   double ms_partition;            /* partition + row grouping */
   double gram_kernel_ms_total;    /* with opts.profile: summed event time of the Gram kernel */
   uint64_t gram_kernel_launches;
+  double ms_iteration_graph;      /* opts.graph: the last replayed iteration, without the fit copy */
+  int tuned_chunk_tiles[GT_MAX_ORDER]; /* gt_autotune: 64-nonzero tiles per chunk, per mode (0: none) */
+  int tuned_variant[GT_MAX_ORDER];     /* gt_autotune: 1 = core operand from L2 chosen */
+  double tuned_ms[GT_MAX_ORDER];       /* gt_autotune: the chosen delta + Gram pass time */
   int delta_path[GT_MAX_ORDER];   /* gt_delta_path of the last update of each mode: a BF16
                                      request that no MFMA kernel covers runs GT_DELTA_FP32 */
 } gt_timings;
@@ -175,6 +185,14 @@ int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core);
  * inverse update (update.cuh:163-194). */
 int gt_mode_update(gt_ctx *ctx, int mode);
 
+/* The project's autotuner (no counterpart in the reference, which fixes its
+ * launch shape): per mode, times the delta + Gram pass at chunk sizes of 1 to
+ * 32 tiles of 64 nonzeros and, for refined BF16 at orders 4-5 where both
+ * apply, with the core operand from LDS and from L2, and keeps the fastest
+ * for later mode updates (gt_timings.tuned_*).  In core only (a no-op out of
+ * core); factors are untouched.  Needs gt_set_init first. */
+int gt_autotune(gt_ctx *ctx);
+
 /* Replaces Reconstruction (reconstruction.cuh:228-277): fit = 1 - ||x - xhat|| / ||x||. */
 int gt_fit(gt_ctx *ctx, double *fit);
 
@@ -217,6 +235,11 @@ int gt_get_recon(gt_ctx *ctx, float *out);
 
 int gt_get_timings(gt_ctx *ctx, gt_timings *t);
 int gt_synchronize(gt_ctx *ctx);
+
+/* The rank the kernels run a rank-R model at: R itself when it is built,
+ * else the next built rank, with zero extra factor columns and core slices
+ * (results as at rank R).  0 when R is outside 1..GT_MAX_RANK. */
+int gt_kernel_rank(int rank);
 
 /* Library version: major*10000 + minor*100 + patch. */
 int gt_version(void);

@@ -52,13 +52,42 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+_use_baseline = False
+_blib = None
+
+
+def use_baseline(on=True):
+    """Routes every oracle call to libbaseline.so, the same restatement built
+    with -O3 -march=native, for timing the CPU baseline (bench.py); off again
+    for checking."""
+    global _use_baseline
+    _use_baseline = bool(on)
+
+
 def lib():
-    global _lib
+    global _lib, _blib
+    if _use_baseline:
+        if _blib is None:
+            path = os.path.join(_HERE, "libbaseline.so")
+            if not os.path.exists(path):
+                build()
+            _blib = _bind(C.CDLL(path))
+        return _blib
     if _lib is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
+        _lib = _bind(C.CDLL(path))
+    return _lib
+
+
+def max_threads():
+    """OpenMP threads the oracle's parallel loops use."""
+    return lib().or_max_threads()
+
+
+def _bind(L):
+    if True:
         L.or_block_dims.argtypes = [C.c_int, u32p, u32p, u32p]
         L.or_block_coord.argtypes = [C.c_int, u32p, C.c_uint64, u32p]
         L.or_block_coord.restype = None
@@ -90,8 +119,9 @@ def lib():
             getattr(L, f"or_als_{sfx}").argtypes = [C.POINTER(OrTensor), fp, ctype, C.c_void_p, fp,
                                                    C.c_double, C.c_int, C.c_double, f64p]
             getattr(L, f"or_als_{sfx}").restype = C.c_int
-        _lib = L
-    return _lib
+        L.or_max_threads.argtypes = []
+        L.or_max_threads.restype = C.c_int
+    return L
 
 
 def eigen_ref():

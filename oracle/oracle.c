@@ -9,8 +9,11 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+
+int or_max_threads(void) { return omp_get_max_threads(); }
 
 /* tensor.tpp:198-213 and optimizer.tpp:359-376 (_RefreshDims / _RefreshBlockDims) */
 int or_block_dims(int order, const uint32_t *dims, const uint32_t *part, uint32_t *bd_out) {

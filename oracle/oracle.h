@@ -57,6 +57,8 @@ typedef struct or_tensor {
 
 /* tensor.tpp:198-213 / optimizer.tpp:359-376: bd = ceil(I/P); -1 when
  * ceil(I/bd) != P (the reference throws). */
+/* OpenMP threads the parallel loops use. */
+int or_max_threads(void);
 int or_block_dims(int order, const uint32_t *dims, const uint32_t *part, uint32_t *bd_out);
 
 /* tensor.tpp:156-174: mixed-radix decode, last axis fastest. */

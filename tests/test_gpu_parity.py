@@ -519,3 +519,63 @@ def test_padded_rank_is_the_embedded_model(R, RK):
         assert a.shape == (dims[m], R)
         assert np.array_equal(a, ak[:, :R]), m
         assert not ak[:, R:].any(), m
+
+
+@pytest.mark.parametrize("prec,N,R,part", [("bf16r", 3, 10, [2, 2, 2]), ("fp32", 4, 6, [1, 2, 1, 1]),
+                                           ("fp32", 3, 20, [1, 1, 1])])
+def test_graph_iteration_equals_eager(prec, N, R, part):
+    """An iteration captured as a HIP graph (kernels only) and replayed must
+    give what the eager launches give, bit for bit, replay after replay, with
+    eager work (factor reads and writes, a reconstruction pass) between the
+    replays; gt_set_init drops the graph and the next iteration recaptures."""
+    print(f"graph vs eager: {prec} N={N} R={R}", flush=True)
+    dims = [120, 100, 80, 60][:N]
+    idx, vals, dims = random_coo(N * 5 + R, dims, 20000, skew=1.1)
+    A, G = random_init(N * 5 + R, dims, R)
+    kw = dict(precision=prec, lam=1e-3)
+    eg = make(idx, vals, dims, R, part, **kw)
+    gr = make(idx, vals, dims, R, part, graph=True, **kw)
+    for e in (eg, gr):
+        e.set_init(A, G)
+    for it in range(4):
+        f_e, f_g = eg.iteration(), gr.iteration()
+        assert f_e == f_g, (it, f_e, f_g)
+        for m in range(N):
+            assert np.array_equal(eg.get_factor(m), gr.get_factor(m)), (it, m)
+        assert eg.fit() == gr.fit()
+        if it == 1:  # new factors through the boundary: the graph is kept (same buffers)
+            a0 = eg.get_factor(0) * np.float32(0.5)
+            eg.set_factor(0, a0)
+            gr.set_factor(0, a0)
+    assert gr.timings()["ms_iteration_graph"] > 0
+    for e in (eg, gr):
+        e.set_init(A, G)  # drops the graph
+    assert eg.iteration() == gr.iteration()
+
+
+@pytest.mark.parametrize("prec,N,R,part", [("bf16r", 3, 10, [2, 2, 2]), ("bf16r", 4, 5, [1, 2, 1, 1]),
+                                           ("fp32", 3, 20, [1, 1, 1])])
+def test_autotune_keeps_parity(prec, N, R, part):
+    """gt_autotune times each mode's delta + Gram pass over chunk sizes (and,
+    for refined BF16 at order 4, the core operand from LDS or from L2) and
+    keeps the fastest.  The factors are untouched by the tuning, and the ALS
+    run that follows agrees with the untuned one to FP64 rounding of the Gram
+    sums and with the FP64 oracle to the precision's bar."""
+    print(f"autotune: {prec} N={N} R={R}", flush=True)
+    dims = [200, 150, 120, 60][:N]
+    e, t, A, G = _setup(N * 3 + R, dims, 60000, R, part, skew=1.1, precision=prec, lam=1e-3,
+                        max_iters=3, tol=0.0)
+    a0 = [e.get_factor(m) for m in range(N)]
+    tuned = e.autotune()
+    assert all(tl in (1, 2, 4, 8, 16, 32) for tl in tuned["chunk_tiles"]), tuned
+    assert all(ms > 0 for ms in tuned["ms"]), tuned
+    for m in range(N):
+        assert np.array_equal(e.get_factor(m), a0[m])
+    fits = e.run()
+    ref = make(t.idx, t.vals.astype(np.float32), [int(d) for d in t.dims], R, part,
+               precision=prec, lam=1e-3, max_iters=3, tol=0.0)
+    ref.set_init([a[: dims[k]].astype(np.float32) for k, a in enumerate(A)], G.astype(np.float32))
+    fits_ref = ref.run()
+    assert np.allclose(fits, fits_ref, rtol=1e-7, atol=0), (fits, fits_ref)
+    f64, _ = oracle.als(t, A, G, 1e-3, 3, 0.0, "f64")
+    assert (np.abs(fits - f64) / np.abs(f64)).max() < (1e-5 if prec == "fp32" else 1e-4)
