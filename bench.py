@@ -65,10 +65,10 @@ HBM_PEAK_GBS = 8000.0
 # the delta + Gram kernel behind each gt_delta_path, as rocprofv3 names it
 _PATH_KERNEL = {"fp32": "k_gram_valu<{R}>", "fp32-wide": "k_gram_fp32<{R}>",
                 "bf16": "k_gram_bf16<{R}, 1>", "bf16r": "k_gram_bf16<{R}, 2>",
-                "bf16r-l2": "k_gram_bf16_o4<{R}, {NS}>"}
+                "bf16r-l2": "k_gram_bf16_o4<{R}, {NS}>", "bf16r-wide": "k_gram_bf16w<{R}>"}
 # the arithmetic each path's delta runs in
 _PATH_DTYPE = {"fp32": "fp32", "fp32-wide": "fp32", "bf16": "bf16", "bf16r": "bf16r",
-               "bf16r-l2": "bf16r"}
+               "bf16r-l2": "bf16r", "bf16r-wide": "bf16r"}
 
 
 def committed_counters(workload, kernel):
@@ -111,7 +111,7 @@ def kernel_bytes_per_nnz(N, R, path):
     indices and the value (no row-grouping entry: the stream is row-sorted),
     the split BF16 copy of the last other factor (16 words per row) on the
     MFMA paths or its FP32 row, and the other factors' FP32 rows."""
-    a_l = 64 if path.startswith("bf16") else 4 * R
+    a_l = 128 if path == "bf16r-wide" else (64 if path.startswith("bf16") else 4 * R)
     return 4 * N + 4 + a_l + (N - 2) * 4 * R
 
 
@@ -248,10 +248,11 @@ def main(argv=None):
         return 2
     cfg = CONFIGS[a.config]
     N, R = len(cfg["dims"]), cfg["rank"]
-    if R > 15 and (a.precision != "fp32" or a.compare):
-        # the BF16 delta is built for R <= 15; above that FP32 is the only path
-        print(f"[bench] R = {R}: timing the FP32 path only", file=sys.stderr, flush=True)
-        a.precision, a.compare = "fp32", ""
+    if R > 15 and a.precision == "bf16":
+        # plain BF16 is built for R <= 15; the wide ranks have the refined delta
+        print(f"[bench] R = {R}: plain BF16 is built for R <= 15, timing bf16r",
+              file=sys.stderr, flush=True)
+        a.precision = "bf16r"
     if a.compare == a.precision:
         a.compare = ""
 

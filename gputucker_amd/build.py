@@ -39,8 +39,8 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["gt_sort.hip", "gt_partition.hip", "gt_gram.hip", "gt_gram_mfma.hip", "gt_solve.hip",
-           "gt_abi.hip"]
+SOURCES = ["gt_sort.hip", "gt_partition.hip", "gt_gram.hip", "gt_gram_mfma.hip", "gt_gram_wide.hip",
+           "gt_solve.hip", "gt_abi.hip"]
 
 
 def _compile(src: str) -> str:

@@ -178,6 +178,94 @@ contract_slots(int slots, const float *__restrict__ A_L, uint32_t iL,
   }
 }
 
+// The per-row Gram accumulation of a chunk with one lane per Gram entry (the
+// wide ranks, R > 15): lane q owns entries q, q + 64, ... of the packed row
+// (upper triangle of B, then C = sum x delta) and walks each tile's
+// nonzeros in order with FP64 products of the FP32 deltas, flushing on every
+// row change: complete rows to the Gram table, the chunk's first and last
+// runs, when they cross its edges, to the partial tables of the fix-up.  The
+// tile is kWave rows of pitch RP in LDS, deltas in columns < R and x in
+// column R; `srow` holds the tiles' row ids.
+template <int R, int RP> struct LaneGram {
+  static constexpr int W = gram_width(R);
+  static constexpr int NE = gram_entries(R);
+  static constexpr int E = (NE + kWave - 1) / kWave; // entries per lane
+  const GramArgs &a;
+  uint64_t c, cb, ce;
+  int lane;
+  uint32_t off[E]; // the entry's two tile columns, p | q << 16 (p = q = 0 past NE)
+  double acc[E];
+  uint32_t cur, last_row;
+  uint64_t run_begin;
+  bool head_open, tail_open;
+
+  __device__ LaneGram(const GramArgs &args, uint64_t chunk, uint64_t b, uint64_t e, int l)
+      : a(args), c(chunk), cb(b), ce(e), lane(l) {
+    // entry ownership: q < R(R+1)/2 -> B(p, p2), p <= p2; else C(p) = delta_p * x
+#pragma unroll
+    for (int qi = 0; qi < E; ++qi) {
+      const int q = lane + qi * kWave;
+      int p = 0, p2 = 0;
+      if (q < R * (R + 1) / 2) {
+        int rem = q;
+        while (rem >= R - p) {
+          rem -= R - p;
+          ++p;
+        }
+        p2 = p + rem;
+      } else if (q < NE) {
+        p = q - R * (R + 1) / 2;
+        p2 = R;
+      }
+      off[qi] = (uint32_t)p | ((uint32_t)p2 << 16);
+      acc[qi] = 0.0;
+    }
+    cur = __builtin_amdgcn_readfirstlane(a.idx_row[cb]);
+    run_begin = cb;
+    // Runs other than the chunk's first and last are whole rows.
+    head_open = a.row_ptr[cur] < cb;
+    last_row = __builtin_amdgcn_readfirstlane(a.idx_row[ce - 1]);
+    tail_open = a.row_ptr[last_row + 1] > ce;
+  }
+
+  __device__ void flush(uint64_t e_end) {
+    const bool open = (run_begin == cb && head_open) || (e_end == ce && tail_open);
+    double *dst;
+    if (!open)
+      dst = a.gram + (size_t)cur * W;
+    else if (run_begin == cb)
+      dst = a.bpart0 + (size_t)c * W;
+    else
+      dst = a.bpart1 + (size_t)c * W;
+#pragma unroll
+    for (int qi = 0; qi < E; ++qi)
+      if (lane + qi * kWave < NE) dst[lane + qi * kWave] = acc[qi];
+  }
+
+  // Adds tile [tb, tb + cnt); tile and srow must be visible to the whole wave.
+  __device__ void add_tile(const float *tile, const uint32_t *srow, uint64_t tb, int cnt) {
+    for (int j = 0; j < cnt; ++j) {
+      const uint32_t row = __builtin_amdgcn_readfirstlane(srow[j]);
+      if (row != cur) {
+        flush(tb + j);
+        cur = row;
+        run_begin = tb + j;
+#pragma unroll
+        for (int qi = 0; qi < E; ++qi) acc[qi] = 0.0;
+      }
+      const float *tr = tile + j * RP;
+#pragma unroll
+      for (int qi = 0; qi < E; ++qi)
+        acc[qi] += (double)tr[off[qi] & 0xFFFFu] * (double)tr[off[qi] >> 16];
+    }
+  }
+
+  __device__ void finish() {
+    flush(ce);
+    if (lane == 0) a.chunk_tail[c] = tail_open ? last_row : 0xFFFFFFFFu;
+  }
+};
+
 constexpr int cpow(int b, int e) { return e == 0 ? 1 : b * cpow(b, e - 1); }
 
 // Whether contract_pair's fully unrolled form is used for NS outer slots:

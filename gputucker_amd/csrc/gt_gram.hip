@@ -54,8 +54,7 @@ int waves_per_block(size_t per_wave) {
 
 template <int R>
 __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a, int wpb) {
-  using Lay = GramLayout<R>;
-  constexpr int W = Lay::W, NE = Lay::NE, E = Lay::E, RP = Lay::RP;
+  constexpr int RP = GramLayout<R>::RP;
   extern __shared__ float s_dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
@@ -66,53 +65,7 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a, int wpb) {
   const uint64_t cb = c * a.chunk;
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
   const uint32_t *__restrict__ rows = a.idx_row;
-
-  // entry ownership: q < R(R+1)/2 -> B(p, p2), p <= p2; else C(p) = delta_p * x
-  int offA[E], offB[E];
-  bool act[E];
-#pragma unroll
-  for (int qi = 0; qi < E; ++qi) {
-    const int q = lane + qi * kWave;
-    act[qi] = q < NE;
-    int p = 0, p2 = 0;
-    if (q < R * (R + 1) / 2) {
-      int rem = q;
-      while (rem >= R - p) {
-        rem -= R - p;
-        ++p;
-      }
-      p2 = p + rem;
-    } else if (q < NE) {
-      p = q - R * (R + 1) / 2;
-      p2 = R;
-    }
-    offA[qi] = p;
-    offB[qi] = p2;
-  }
-  double acc[E];
-#pragma unroll
-  for (int qi = 0; qi < E; ++qi) acc[qi] = 0.0;
-
-  uint32_t cur = __builtin_amdgcn_readfirstlane(rows[cb]);
-  uint64_t run_begin = cb;
-  // Runs other than the chunk's first and last are whole rows.
-  const bool head_open = a.row_ptr[cur] < cb;
-  const uint32_t last_row = __builtin_amdgcn_readfirstlane(rows[ce - 1]);
-  const bool tail_open = a.row_ptr[last_row + 1] > ce;
-
-  auto flush = [&](uint32_t row, uint64_t b, uint64_t e_end) {
-    const bool open = (b == cb && head_open) || (e_end == ce && tail_open);
-    double *dst;
-    if (!open)
-      dst = a.gram + (size_t)row * W;
-    else if (b == cb)
-      dst = a.bpart0 + (size_t)c * W;
-    else
-      dst = a.bpart1 + (size_t)c * W;
-#pragma unroll
-    for (int qi = 0; qi < E; ++qi)
-      if (act[qi]) dst[lane + qi * kWave] = acc[qi];
-  };
+  LaneGram<R, RP> gram(a, c, cb, ce, lane);
 
   for (uint64_t tb = cb; tb < ce; tb += kWave) {
     const uint64_t e = tb + lane;
@@ -135,24 +88,10 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a, int wpb) {
 #pragma unroll
       for (int r = 0; r < R; ++r) a.delta_out[(tb + lane) * R + r] = tile[lane * RP + r];
     }
-    for (int j = 0; j < cnt; ++j) {
-      const uint32_t row = __builtin_amdgcn_readfirstlane(s_row[j]);
-      if (row != cur) {
-        flush(cur, run_begin, tb + j);
-        cur = row;
-        run_begin = tb + j;
-#pragma unroll
-        for (int qi = 0; qi < E; ++qi) acc[qi] = 0.0;
-      }
-      const float *tr = tile + j * RP;
-#pragma unroll
-      for (int qi = 0; qi < E; ++qi)
-        if (act[qi]) acc[qi] += (double)tr[offA[qi]] * (double)tr[offB[qi]];
-    }
+    gram.add_tile(tile, s_row, tb, cnt);
     wave_sync();
   }
-  flush(cur, run_begin, ce);
-  if (lane == 0) a.chunk_tail[c] = tail_open ? last_row : 0xFFFFFFFFu;
+  gram.finish();
 }
 
 // Fix-up of rows that straddle chunks, in two levels so that a row spread

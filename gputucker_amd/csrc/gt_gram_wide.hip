@@ -1,0 +1,223 @@
+// gt_gram_wide.hip -- the refined BF16 MFMA delta for the wide ranks of an
+// order-3 tensor (16 <= R <= 32), fused with the per-row Gram, and the split
+// BF16 copy of a factor it gathers.
+//
+// With L the last mode other than n and O the remaining one (s_m[o] =
+// A_O[i_O(m)][o]), the delta of nonzero m is, as in delta.cuh:62-90,
+//   delta[m][r] = sum_o s_m[o] sum_jL A_L[i_L(m)][jL] G[o][jL][r].
+// Stage 1 (MFMA): for each o, U_o[m][r] = sum_jL A_L[m][jL] G[o][jL][r] on
+// v_mfma_f32_16x16x32_bf16, the core as the A operand (16 output columns r
+// per tile, MT = ceil(R/16) tiles), the factor rows as the B operand (16
+// nonzeros), K = jL (32 slots, zero above R), refined: with a = a_hi + a_lo
+// and G = G_hi + G_lo (each part BF16), three accumulating MFMAs add
+// a_hi.G_hi, a_lo.G_hi and a_hi.G_lo.  Lane (q, g) then holds
+// U_o[16mb + q][16mt + 4g + i], i < 4.
+// Stage 2 (VALU FP32): delta[m][r] += U_o[m][r] s_m[o], one s value per block
+// for the lane's own nonzero column, read from this wave's LDS.
+// The core operand (128 KiB for R = 32, more than LDS holds beside the
+// waves) is read from global memory, where it stays in L2, one outer index
+// ahead.  A tile's stage-1 and stage-2 work (768 MFMAs and 1024 FMAs per lane
+// at R = 32) is long beside its gathers, so the gathers are not prefetched
+// across tiles: the other waves of the SIMD run while one waits.  The tile
+// of deltas then goes through LDS to the lane-per-entry FP64 Gram
+// (LaneGram), which reuses the LDS of the s values.
+#include "gt_bf16.hpp"
+#include "gt_contract.hpp"
+
+namespace gt {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+
+template <int R> struct WideLayout {
+  static constexpr int MT = (R + 15) / 16;           // 16-column output tiles
+  static constexpr int RP = ((R + 1) + 3) / 4 * 4;   // delta tile pitch: 16-byte rows, x at R
+  static constexpr int S_FLOATS = R * kWave;         // s values, [o][nonzero]
+  static constexpr int T_FLOATS = kWave * RP;        // delta tile, [nonzero][column]
+  static constexpr int BUF = S_FLOATS > T_FLOATS ? S_FLOATS : T_FLOATS;
+  static constexpr int PER_WAVE = BUF + kWave;       // + the tile's row ids
+};
+
+template <int R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_gram_bf16w(GramArgs a, int wpb) {
+  static_assert(R >= 16 && R <= 32, "the wide kernel covers 16 <= R <= 32");
+  using Lay = WideLayout<R>;
+  constexpr int MT = Lay::MT, RP = Lay::RP, RS = row_stride(R);
+  extern __shared__ __align__(16) float dyn[];
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
+  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
+  if (c >= a.chunk_end) return;
+  float *buf = dyn + (size_t)w * Lay::PER_WAVE; // s values, then the delta tile
+  uint32_t *s_row = reinterpret_cast<uint32_t *>(buf + Lay::BUF);
+  const uint64_t cb = c * a.chunk;
+  const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
+  const int q = lane & 15, g = lane >> 4;
+  // core operand words, [o][mt][lane][hi, lo] (16 bytes each)
+  const u4 *tab = reinterpret_cast<const u4 *>(a.Gb);
+  LaneGram<R, RP> gram(a, c, cb, ce, lane);
+
+  for (uint64_t tb = cb; tb < ce; tb += kWave) {
+    const uint64_t e = tb + lane;
+    const bool valid = e < ce;
+    const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
+    const uint64_t ee = valid ? e : cb; // in-bounds position for padding lanes
+    const uint32_t row = a.idx_row[ee];
+    const uint32_t iL = a.idx_L[ee];
+    const uint32_t io = a.idx_O[0][ee];
+    const float x = valid ? a.st.val[ee] : 0.f;
+    // lane (q, g) takes, for block mb, the split words of nonzero 16 mb + q:
+    // hi and lo of columns 8g..8g+7 of its A_L row
+    u4 ah[4], al[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const uint32_t r = (uint32_t)__shfl((int)iL, mb * 16 + q);
+      const u4 *p = reinterpret_cast<const u4 *>(a.As_L + (size_t)r * kSplitWordsWide + 8 * g);
+      ah[mb] = p[0];
+      al[mb] = p[1];
+    }
+    {
+      float ro[RS];
+      load_row<R>(a.A_O[0], io, ro);
+#pragma unroll
+      for (int o = 0; o < R; ++o) buf[o * kWave + lane] = ro[o];
+    }
+    s_row[lane] = valid ? row : 0xFFFFFFFFu;
+    u4 nh[MT], nl[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      nh[mt] = tab[((size_t)mt * kWave + lane) * 2];
+      nl[mt] = tab[((size_t)mt * kWave + lane) * 2 + 1];
+    }
+    wave_sync();
+    f4 dacc[4][MT];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) dacc[mb][mt] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int o = 0; o < R; ++o) {
+      u4 gh[MT], gl[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        gh[mt] = nh[mt];
+        gl[mt] = nl[mt];
+      }
+      if (o + 1 < R) { // the next outer index's core operand, from L2
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          nh[mt] = tab[(((size_t)(o + 1) * MT + mt) * kWave + lane) * 2];
+          nl[mt] = tab[(((size_t)(o + 1) * MT + mt) * kWave + lane) * 2 + 1];
+        }
+      }
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const float sv = buf[o * kWave + mb * 16 + q];
+        const b8 bh = __builtin_bit_cast(b8, ah[mb]), bl = __builtin_bit_cast(b8, al[mb]);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          const b8 th = __builtin_bit_cast(b8, gh[mt]), tl = __builtin_bit_cast(b8, gl[mt]);
+          f4 u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(th, bh, (f4{0.f, 0.f, 0.f, 0.f}), 0, 0, 0);
+          u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(th, bl, u, 0, 0, 0);
+          u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(tl, bh, u, 0, 0, 0);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dacc[mb][mt][i] = __builtin_fmaf(u[i], sv, dacc[mb][mt][i]);
+        }
+      }
+    }
+    wave_sync(); // every s value read before the delta tile overwrites them
+    // lane (q, g) holds delta[16mb + q][16mt + 4g + i]: four columns per
+    // 16-byte store; the column group holding R gets x over column R after
+    float *tile = buf;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+        if (16 * mt + 4 * g < R)
+          *reinterpret_cast<f4 *>(tile + (mb * 16 + q) * RP + 16 * mt + 4 * g) = dacc[mb][mt];
+    tile[lane * RP + R] = x;
+    wave_sync();
+    if (a.delta_out != nullptr && lane < cnt) { // test hook, uniform branch
+#pragma unroll
+      for (int r = 0; r < R; ++r) a.delta_out[(tb + lane) * R + r] = tile[lane * RP + r];
+    }
+    gram.add_tile(tile, s_row, tb, cnt);
+    wave_sync();
+  }
+  gram.finish();
+}
+
+// Row i of the wide split copy: for each lane group g, the BF16 words of
+// hi(a[8g..8g+7]) (four words) then lo(a[8g..8g+7]), zero at and beyond R,
+// lo = bf16(a - hi).  One thread per (row, group).
+template <int R>
+__global__ void __launch_bounds__(256) k_split_wide(const float *__restrict__ A,
+                                                    uint32_t *__restrict__ As, uint64_t rows) {
+  constexpr int RS = row_stride(R);
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i = t >> 2;
+  const int g = (int)(t & 3);
+  if (i >= rows) return;
+  const float *row = A + i * RS;
+  float v[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = (8 * g + k < R) ? row[8 * g + k] : 0.f;
+  u4 hi, lo;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t h = pk_bf16(v[2 * k], v[2 * k + 1]);
+    hi[k] = h;
+    lo[k] = pk_bf16(v[2 * k] - bf16_low_f(h), v[2 * k + 1] - bf16_high_f(h));
+  }
+  u4 *dst = reinterpret_cast<u4 *>(As + i * kSplitWordsWide + 8 * g);
+  dst[0] = hi;
+  dst[1] = lo;
+}
+
+template <int R> void wide_impl(const GramArgs &a, hipStream_t s) {
+  const size_t per_wave = (size_t)WideLayout<R>::PER_WAVE * sizeof(float);
+  const int wpb = 4;
+  if (per_wave * wpb > lds_limit())
+    throw std::invalid_argument("wide BF16 Gram: LDS per workgroup exceeds the device's");
+  if (a.chunk_end <= a.chunk_begin) return;
+  const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
+  hipLaunchKernelGGL(k_gram_bf16w<R>, dim3(grid), dim3(64 * wpb), per_wave * wpb, s, a, wpb);
+}
+
+} // namespace
+
+bool gram_wide_supported(int R, int order) { return order == 3 && R >= 16 && R <= 32; }
+
+void launch_gram_bf16_wide(int R, const GramArgs &a, hipStream_t s) {
+  switch (R) {
+#define GT_CASE(r)                                                                               \
+  case r:                                                                                        \
+    wide_impl<r>(a, s);                                                                          \
+    break;
+    GT_CASE(16) GT_CASE(20) GT_CASE(24) GT_CASE(28) GT_CASE(30) GT_CASE(32)
+#undef GT_CASE
+  default:
+    throw std::invalid_argument("the wide BF16 delta is built for R = 16, 20, 24, 28, 30, 32");
+  }
+  GT_HIP(hipGetLastError());
+}
+
+void launch_split_wide(int R, const float *A, uint64_t rows, uint32_t *As, hipStream_t s) {
+  if (rows == 0) return;
+  const unsigned grid = (unsigned)((rows * 4 + 255) / 256);
+  switch (R) {
+#define GT_CASE(r)                                                                               \
+  case r:                                                                                        \
+    hipLaunchKernelGGL(k_split_wide<r>, dim3(grid), dim3(256), 0, s, A, As, rows);              \
+    break;
+    GT_CASE(16) GT_CASE(20) GT_CASE(24) GT_CASE(28) GT_CASE(30) GT_CASE(32)
+#undef GT_CASE
+  default:
+    throw std::invalid_argument("the wide split copy is built for R = 16, 20, 24, 28, 30, 32");
+  }
+  GT_HIP(hipGetLastError());
+}
+
+} // namespace gt

@@ -225,6 +225,19 @@ int launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t
 constexpr int kSplitWords = 16;
 void launch_split_factor(int R, const float *A, uint64_t rows, uint32_t *As, hipStream_t s);
 
+// The refined BF16 MFMA delta of the wide ranks (order 3, 16 <= R <= 32,
+// built for R = 16, 20, 24, 28, 30, 32) with the lane-per-entry FP64 Gram.
+// It gathers a wide split copy of A_L (kSplitWordsWide words per row: per
+// lane group g, hi then lo of columns 8g..8g+7) and reads its core operand,
+// [o][mt][lane][hi 8, lo 8] BF16 words, from global memory.
+constexpr int kSplitWordsWide = 32;
+bool gram_wide_supported(int R, int order);
+void launch_gram_bf16_wide(int R, const GramArgs &a, hipStream_t s);
+void launch_split_wide(int R, const float *A, uint64_t rows, uint32_t *As, hipStream_t s);
+__host__ __device__ constexpr size_t gb_wide_words(int R) { // uint16 words of the core table
+  return (size_t)R * ((R + 15) / 16) * kWave * 16;
+}
+
 struct SolveArgs {
   const double *gram;
   const uint64_t *row_ptr;

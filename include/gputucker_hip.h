@@ -55,7 +55,9 @@ typedef enum gt_delta_path {
   GT_DELTA_FP32_WIDE = 2, /* FP32 VALU delta, lane-per-entry FP64 Gram (R > 15) */
   GT_DELTA_BF16 = 3,      /* BF16 MFMA delta, core table in LDS */
   GT_DELTA_BF16R = 4,     /* refined BF16 MFMA delta, core table in LDS */
-  GT_DELTA_BF16R_L2 = 5   /* refined BF16 MFMA delta, core operand from L2 (orders 4-5) */
+  GT_DELTA_BF16R_L2 = 5,  /* refined BF16 MFMA delta, core operand from L2 (orders 4-5) */
+  GT_DELTA_BF16R_WIDE = 6 /* refined BF16 MFMA delta of order 3 at 16 <= R <= 32, core operand
+                             from L2, lane-per-entry FP64 Gram */
 } gt_delta_path;
 
 typedef struct gt_opts {

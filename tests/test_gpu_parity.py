@@ -108,7 +108,9 @@ def _stream_positions(t, idx_stream):
 
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("bf16r", 1e-4), ("bf16", 2e-2)])
 @pytest.mark.parametrize("N,R,part", [(3, 10, [2, 1, 2]), (3, 13, [1, 1, 1]), (4, 8, [1, 2, 1, 1]),
-                                      (5, 5, [1, 1, 1, 2, 1]), (2, 6, [2, 1]), (3, 20, [1, 1, 1])])
+                                      (5, 5, [1, 1, 1, 2, 1]), (2, 6, [2, 1]), (3, 20, [1, 1, 1]),
+                                      (3, 16, [2, 1, 1]), (3, 30, [1, 1, 1]), (3, 32, [1, 2, 1]),
+                                      (3, 17, [1, 1, 1])])
 def test_hot_path_delta_parity(prec, tol, N, R, part):
     """The deltas the mode update's own delta + Gram kernel forms (read back
     through a test hook that stores each tile before its Gram), against the
@@ -116,9 +118,11 @@ def test_hot_path_delta_parity(prec, tol, N, R, part):
     the largest entry of that nonzero's delta.  Covers the FP32 VALU delta
     (two tiles per pass, and the R > 15 kernel), the refined BF16 MFMA delta
     (order 3 from the LDS table, with the second MFMA at R = 13, and orders 4
-    and 5 with the core operand from L2) and plain BF16."""
-    if R > 15 and prec != "fp32":
-        pytest.skip("the BF16 delta covers R <= 15")
+    and 5 with the core operand from L2, and the wide ranks 16..32 of order 3
+    with the core operand from L2; R = 17 runs zero-padded at 20) and plain
+    BF16."""
+    if R > 15 and prec == "bf16":
+        pytest.skip("plain BF16 covers R <= 15")
     dims = [90, 70, 50, 30, 20][:N]
     e, t, A, G = _setup(N * 17 + R, dims, 8000, R, part, skew=1.2, precision=prec, chunk_tiles=1)
     scale_n = max(1, R ** (N - 1) / 100)
@@ -267,7 +271,8 @@ def test_error_paths():
 @pytest.mark.parametrize("N,R,part", [(3, 10, [2, 1, 2]), (4, 5, [1, 2, 1, 1]), (2, 8, [1, 3]),
                                       (3, 13, [2, 1, 2]), (3, 15, [1, 1, 1]), (3, 7, [1, 2, 1]),
                                       (2, 13, [2, 1]), (4, 7, [1, 2, 1, 1]), (4, 9, [1, 1, 2, 1]),
-                                      (5, 4, [1, 1, 1, 2, 1]), (5, 5, [1, 1, 1, 2, 1])])
+                                      (5, 4, [1, 1, 1, 2, 1]), (5, 5, [1, 1, 1, 2, 1]),
+                                      (3, 20, [2, 1, 2]), (3, 32, [1, 1, 2]), (3, 24, [1, 2, 1])])
 def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
     """BF16 MFMA delta (plain and with the split-BF16 refinement; above
     R = 10 the refinement takes a second MFMA per tile; at order 4 with
@@ -276,6 +281,8 @@ def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
     R = 8) and the
     FP32 VALU delta (two tiles per pass) feeding the MFMA Gram, against the
     FP64 oracle's B and C."""
+    if R > 15 and prec == "bf16":
+        pytest.skip("plain BF16 covers R <= 15")
     dims = [90, 70, 50, 30, 20][:N]
     e, t, A, G = _setup(N * 13 + R, dims, 12000, R, part, skew=1.2, precision=prec,
                         chunk_tiles=1)
@@ -310,13 +317,16 @@ def test_gram_parity_high_order_wide_rank(prec, N, R, part):
         assert (np.abs(Cg - Cv) / scc).max() < 1e-5
 
 
-@pytest.mark.parametrize("R", [10, 13])
+@pytest.mark.parametrize("R", [10, 13, 20, 30])
 @pytest.mark.parametrize("prec,tol", [("bf16r", 1e-4), ("bf16", 2e-3), ("fp32", 1e-5)])
 def test_fit_vs_fp64_oracle_by_precision(prec, tol, R):
     """Fit after 5 ALS iterations, relative to the FP64 oracle: the BF16 path
     with refinement must stay within 1e-4, the FP32 path within 1e-5.  At
     R = 13 the refinement takes two MFMAs per block and the row solves write
-    the split copy the next mode's delta gathers."""
+    the split copy the next mode's delta gathers; at R = 20 and 30 the wide
+    kernel runs and a split pass after each solve rebuilds its copy."""
+    if R > 15 and prec == "bf16":
+        pytest.skip("plain BF16 covers R <= 15")
     dims = [200, 150, 120]
     e, t, A, G = _setup(17, dims, 40000, R, [2, 2, 2], skew=1.1, lam=1e-3, max_iters=5,
                         tol=0.0, precision=prec)
@@ -386,8 +396,6 @@ def test_iteration_fit_matches_reconstruction(prec, tol, N, R, part):
     """gt_iteration takes the fit from the last mode's row solves
     (|x|^2 + sum_i a_i^T G_i a_i - 2 a_i^T C_i); gt_fit runs the
     reconstruction pass over the nonzeros.  Both describe the same factors."""
-    if prec == "bf16r" and R > 15:
-        pytest.skip("BF16 delta covers R <= 15")
     dims = [120, 100, 80, 60][:N]
     e, t, A, G = _setup(N * 7 + R, dims, 20000, R, part, skew=1.1, precision=prec)
     for _ in range(2):
@@ -429,16 +437,19 @@ def test_rank32_sample():
                              32, [1, 1, 1], "fp32", 6, 1e-5)
 
 
-def test_rank32_heavy_row_sample():
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16r", 1e-4)])
+def test_rank32_heavy_row_sample(prec, tol):
     """The rank-32 workload's row skew: 3e5 nonzeros with Zipf(1.5) rows in
     mode 0 (its largest row holds about 1.15e5 nonzeros, spread over some
     1800 chunks, so the two-level fix-up sums it), R = 32, one ALS iteration
-    against the FP64 oracle."""
-    print("rank 32 heavy row: 5e4 x 5e4 x 5e3, 3e5 nnz, Zipf(1.5) in mode 0", flush=True)
+    against the FP64 oracle: the FP32 VALU delta and the wide refined BF16
+    MFMA delta."""
+    print(f"rank 32 heavy row: 5e4 x 5e4 x 5e3, 3e5 nnz, Zipf(1.5) in mode 0, {prec}", flush=True)
     e = _one_iteration_vs_oracle([50_000, 50_000, 5_000], 300_000,
                                  [("zipf", 1.5), "uniform", "uniform"], "uniform", 32, [1, 1, 1],
-                                 "fp32", 6, 1e-5)
+                                 prec, 6, tol)
     assert e.info()["nnz"] == 300_000
+    assert e.timings()["delta_path"] == ["fp32-wide" if prec == "fp32" else "bf16r-wide"] * 3
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16r"])
