@@ -51,6 +51,12 @@ CONFIGS = {
     "c5s": dict(name="5-order 1e4^4 x 200, 2e8 nnz uniform, R=5", dims=[10_000] * 4 + [200],
                 nnz=200_000_000, modes=None, values="uniform", rank=5, part=None, iters=3,
                 lam=None),
+    # the out-of-core workload's shape: 5e4^4 x 200 (71-bit coordinates),
+    # R = 5, under a forced 16 GB device budget, with 2e8 of its 2e9 nonzeros;
+    # the partition and every mode update stream from pinned host memory
+    "c5": dict(name="5-order 5e4^4 x 200, 2e8 nnz uniform, R=5, 16 GB device budget",
+               dims=[50_000] * 4 + [200], nnz=200_000_000, modes=None, values="uniform", rank=5,
+               part=None, iters=2, lam=None, budget=16_000_000_000),
     # R = 32 (the FP32 delta and the lane-per-entry Gram) with power-law rows
     # in mode 0; Zipf(1.5) in every mode would need about 2e10 draws for 3e8
     # distinct coordinates
@@ -219,7 +225,9 @@ def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup
                 solve_ms=statistics.mean(tm["ms_solve"]), delta_path=tm["delta_path"],
                 launches=launches, total=total, setup_s=setup_s,
                 partition_ms=tm["ms_partition"], part=[int(p) for p in info["part"]],
-                tuned_chunk_tiles=tm["tuned_chunk_tiles"], graph=graph)
+                tuned_chunk_tiles=tm["tuned_chunk_tiles"], graph=graph,
+                partition_path=info["partition_path"],
+                stream_bytes_per_nnz=info["stream_bytes_per_nnz"])
 
 
 def main(argv=None):
@@ -278,7 +286,7 @@ def main(argv=None):
     runs = {}
     for prec in [p for p in (a.compare, a.precision) if p]:
         r = run_engine(gt, idx, vals, dims, A, G, cfg["part"], R, lam, prec, steps, warmup,
-                       a.chunk_tiles, a.budget, a.graph, a.autotune)
+                       a.chunk_tiles, a.budget or cfg.get("budget", 0), a.graph, a.autotune)
         runs[prec] = r
         print(f"[bench] {prec}: t_mode {r['t_mode'] * 1e3:.3f} ms ({r['value']:.3e} nnz/s), "
               f"delta path {r['delta_path']}, gram kernel {r['kernel_ms']:.3f} ms avg over "
@@ -317,7 +325,9 @@ def main(argv=None):
                    "delta_path": head["delta_path"], "graph": a.graph,
                    "chunk_tiles": head["tuned_chunk_tiles"] if a.autotune else a.chunk_tiles,
                    "parallelism": "single GPU",
-                   "out_of_core": bool(head["out_of_core"])},
+                   "out_of_core": bool(head["out_of_core"]),
+                   "device_budget": a.budget or cfg.get("budget", 0),
+                   "partition_from_host": bool(head["partition_path"])},
         "fit": head["fit"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -332,6 +342,10 @@ def main(argv=None):
                      "limiter": ctr.get("limiter") if ctr else None},
         "cpu_baseline": cpu,
     }
+    if head["out_of_core"]:  # the host-link rate of the staged streams (never `value`)
+        out["pcie"] = {"host_bytes_per_nnz": head["stream_bytes_per_nnz"],
+                       "h2d_gbs": head["stream_bytes_per_nnz"] * idx.shape[1] / head["t_mode"] / 1e9,
+                       "partition_s": head["partition_ms"] / 1e3}
     if a.compare and a.compare in runs:
         base = runs[a.compare]
         out["compare"] = {"precision": a.compare, "value": base["value"],

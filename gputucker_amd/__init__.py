@@ -56,7 +56,8 @@ class GtPartitionInfo(C.Structure):
         ("empty_blocks", C.c_uint64), ("norm", C.c_double), ("partition_type", C.c_int),
         ("stream_count", C.c_int), ("avail_nnz_per_task", C.c_uint64),
         ("out_of_core", C.c_int), ("segment_nnz", C.c_uint64),
-        ("stream_bytes_per_nnz", C.c_uint32),
+        ("stream_bytes_per_nnz", C.c_uint32), ("partition_path", C.c_int),
+        ("partition_pieces", C.c_uint64),
     ]
 
 
@@ -257,7 +258,8 @@ class Engine:
                     max_block_nnz=i.max_block_nnz, empty_blocks=i.empty_blocks, norm=i.norm,
                     type=i.partition_type, streams=i.stream_count, avail_nnz=i.avail_nnz_per_task,
                     out_of_core=i.out_of_core, segment_nnz=i.segment_nnz,
-                    stream_bytes_per_nnz=i.stream_bytes_per_nnz)
+                    stream_bytes_per_nnz=i.stream_bytes_per_nnz, partition_path=i.partition_path,
+                    partition_pieces=i.partition_pieces)
 
     def get_block(self, b, n):
         idx = np.zeros((self.order, max(n, 1)), np.uint32)

@@ -198,6 +198,24 @@ struct gt_ctx {
   bool have_coo = false;
   DevBuf<uint32_t> in_idx[GT_MAX_ORDER];
   DevBuf<float> in_val;
+  // a device budget below the in-core partition's footprint keeps the input
+  // in pinned host memory, and gt_partition builds the per-mode streams from
+  // it bucket by bucket, straight into the host streams
+  bool host_input = false;
+  bool host_partitioned = false; // the last partition ran bucketed from host memory
+  uint64_t partition_pieces = 0; // pieces it built (over all modes)
+  uint32_t *h_in_idx[GT_MAX_ORDER] = {};
+  float *h_in_val = nullptr;
+  void release_host_input() {
+    for (auto &p : h_in_idx)
+      if (p) {
+        (void)hipHostFree(p);
+        p = nullptr;
+      }
+    if (h_in_val) (void)hipHostFree(h_in_val);
+    h_in_val = nullptr;
+    host_input = false;
+  }
   // partition
   bool partitioned = false;
   Plan plan;
@@ -343,6 +361,7 @@ struct gt_ctx {
         for (auto e : em)
           if (e) (void)hipEventDestroy(e);
       release_host_streams();
+      release_host_input();
       if (s_copy) {
         (void)hipStreamSynchronize(s_copy);
         (void)hipStreamDestroy(s_copy);
@@ -549,31 +568,111 @@ void ensure_work(gt_ctx *c) {
 // it is an offset).
 uint64_t pack_words(int bits, uint64_t n) { return bits == 16 ? (n + 1) / 2 : 3 * ((n + 3) / 4); }
 
-void maybe_stream(gt_ctx *c) {
-  const uint64_t budget = c->o.device_budget;
-  if (budget == 0) return;
-  const int N = c->N, R = c->R, W = gram_width(R);
-  const uint64_t nnz = c->nnz;
-  const uint64_t movable = nnz * 4ull * (uint64_t)(N * (N + 1) + (N + 2) + N);
+// Device bytes the ALS keeps whatever the layout: Gram tables, chunk
+// partials, factors and their split copies, row pointers, cores, and 1 MiB
+// of slack for small buffers.
+uint64_t resident_fixed_bytes(const gt_ctx *c) {
+  const int N = c->N, R = c->R;
   uint64_t fixed = (c->gram.n + c->bpart0.n + c->bpart1.n) * sizeof(double) + c->chunk_tail.n * 4;
+  const uint64_t split = c->o.precision == GT_PREC_FP32 ? 0
+                         : (R <= 15 ? kSplitWords : kSplitWordsWide);
   for (int n = 0; n < N; ++n)
-    fixed += c->rows[n] * (uint64_t)row_stride(R) * 4 + (c->rows[n] + 1) * 8;
+    fixed += c->rows[n] * ((uint64_t)row_stride(R) + split) * 4 + (c->rows[n] + 1) * 8;
   fixed += (uint64_t)N * ipow(R, N) * 4 * 2 + (1u << 20);
-  if (movable + fixed <= budget) return; // fits: stay resident
-  if (fixed >= budget) throw GtError(GT_ERR_MEMORY, "device budget is below the per-row state");
-  // packing needs every index below 2^24; GT_STREAM_PACK=0 turns it off
+  return fixed;
+}
+
+// Whether the host streams hold packed indices (every factor at most 2^24
+// rows; GT_STREAM_PACK=0 turns it off), and the bits of each mode's indices.
+bool choose_packing(gt_ctx *c) {
+  const int N = c->N;
   const char *pack_env = std::getenv("GT_STREAM_PACK");
   bool packed = !(pack_env && pack_env[0] == '0');
   for (int k = 0; k < N; ++k) packed = packed && c->rows[k] <= (1ull << 24);
+  for (int k = 0; k < N; ++k) c->pack_bits[k] = !packed ? 0 : c->rows[k] <= (1u << 16) ? 16 : 24;
+  c->packed = packed;
+  c->stream_bytes = 4u * (N + 1);
+  if (packed) {
+    uint32_t most = 0;
+    for (int n = 0; n < N; ++n) {
+      uint32_t b = 4;
+      for (int k = 0; k < N; ++k)
+        if (k != n) b += (uint32_t)c->pack_bits[k] / 8;
+      most = std::max(most, b);
+    }
+    c->stream_bytes = most;
+  }
+  return packed;
+}
+
+// The ALS side of the out-of-core pipeline once the host streams exist: the
+// staged segment size (a multiple of the chunk) from the device bytes left,
+// two staging buffers, the copy stream and its events.
+void setup_staging(gt_ctx *c, uint64_t fixed) {
+  const int N = c->N;
+  const uint64_t budget = c->o.device_budget, nnz = c->nnz;
   // per nonzero and staging buffer: the unpacked indices and the value, plus
   // the packed words when packing
-  const uint64_t stage_bytes = 4ull * (N + 1) + (packed ? 3ull * N : 0);
+  const uint64_t stage_bytes = 4ull * (N + 1) + (c->packed ? 3ull * N : 0);
   uint64_t seg = (budget - fixed) / (2ull * stage_bytes);
   if (c->plan.avail) seg = std::min(seg, c->plan.avail);
+  // at least four segments, so the copies of later ones run under the
+  // kernels of earlier ones (one segment would serialise copy and kernel)
+  seg = std::min(seg, std::max<uint64_t>((nnz + 3) / 4, c->chunk));
   seg = (seg / c->chunk) * c->chunk;
   if (seg == 0) throw GtError(GT_ERR_MEMORY, "device budget too small for one staged chunk");
-  (void)W;
-  for (int k = 0; k < N; ++k) c->pack_bits[k] = !packed ? 0 : c->rows[k] <= (1u << 16) ? 16 : 24;
+  for (int b = 0; b < 2; ++b) {
+    for (int k = 0; k < N; ++k) {
+      c->stage_idx[b][k].alloc(seg);
+      if (c->packed) c->stage_pack[b][k].alloc(3 * (seg / 4));
+    }
+    c->stage_val[b].alloc(seg);
+    if (!c->ev_copy[b]) GT_HIP(hipEventCreateWithFlags(&c->ev_copy[b], hipEventDisableTiming));
+    if (!c->ev_comp[b]) GT_HIP(hipEventCreateWithFlags(&c->ev_comp[b], hipEventDisableTiming));
+  }
+  if (!c->s_copy) GT_HIP(hipStreamCreateWithFlags(&c->s_copy, hipStreamNonBlocking));
+  const uint64_t nseg = (nnz + seg - 1) / seg;
+  c->err_part.alloc(std::max<uint64_t>(4096, nseg * kErrPartialsPerSegment));
+  c->seg_nnz = seg;
+  c->streaming = true;
+}
+
+// Device bytes per nonzero at the peak of the in-core partition: the input
+// copy, the sort's keys, values and ping-pong buffers, the block-ordered COO
+// with its block ids, and every mode's stream.
+uint64_t incore_partition_bytes_per_nnz(int N) {
+  return (4ull * N + 4) + 16 + (4ull * N + 8) + (uint64_t)N * (4ull * N + 4);
+}
+
+// resident_fixed_bytes before the partition, from the mode sizes (the rows
+// of a partition exceed them by less than P_n) and the default chunk size.
+uint64_t fixed_estimate(const gt_ctx *c, uint64_t nnz, const uint32_t *dims) {
+  const int N = c->N, R = c->R, W = gram_width(R);
+  uint64_t tiles = (uint64_t)c->o.chunk_tiles;
+  if (tiles == 0) tiles = std::min<uint64_t>(std::max<uint64_t>(nnz / (kTile * 16384), 1), 16);
+  const uint64_t nchunks = (nnz + tiles * kTile - 1) / (tiles * kTile) + 1;
+  const uint64_t split = c->o.precision == GT_PREC_FP32 ? 0
+                         : (R <= 15 ? kSplitWords : kSplitWordsWide);
+  uint64_t max_rows = 1, fixed = 0;
+  for (int n = 0; n < N; ++n) {
+    max_rows = std::max<uint64_t>(max_rows, dims[n]);
+    fixed += (uint64_t)dims[n] * ((uint64_t)row_stride(R) + split) * 4 + ((uint64_t)dims[n] + 1) * 8;
+  }
+  fixed += max_rows * W * 8 + nchunks * (2ull * W * 8 + 4);
+  fixed += (uint64_t)N * ipow(R, N) * 4 * 2 + (1u << 20);
+  return fixed;
+}
+
+void maybe_stream(gt_ctx *c) {
+  const uint64_t budget = c->o.device_budget;
+  if (budget == 0) return;
+  const int N = c->N;
+  const uint64_t nnz = c->nnz;
+  const uint64_t movable = nnz * 4ull * (uint64_t)(N * (N + 1) + (N + 2) + N);
+  const uint64_t fixed = resident_fixed_bytes(c);
+  if (movable + fixed <= budget) return; // fits: stay resident
+  if (fixed >= budget) throw GtError(GT_ERR_MEMORY, "device budget is below the per-row state");
+  const bool packed = choose_packing(c);
   DevBuf<uint32_t> tmp;
   if (packed) tmp.alloc(3 * ((nnz + 3) / 4));
   for (int n = 0; n < N; ++n) {
@@ -606,32 +705,7 @@ void maybe_stream(gt_ctx *c) {
   c->coo_val.reset();
   c->coo_blk.reset();
   tmp.reset();
-  for (int b = 0; b < 2; ++b) {
-    for (int k = 0; k < N; ++k) {
-      c->stage_idx[b][k].alloc(seg);
-      if (packed) c->stage_pack[b][k].alloc(3 * (seg / 4));
-    }
-    c->stage_val[b].alloc(seg);
-    if (!c->ev_copy[b]) GT_HIP(hipEventCreateWithFlags(&c->ev_copy[b], hipEventDisableTiming));
-    if (!c->ev_comp[b]) GT_HIP(hipEventCreateWithFlags(&c->ev_comp[b], hipEventDisableTiming));
-  }
-  if (!c->s_copy) GT_HIP(hipStreamCreateWithFlags(&c->s_copy, hipStreamNonBlocking));
-  const uint64_t nseg = (nnz + seg - 1) / seg;
-  c->err_part.alloc(std::max<uint64_t>(4096, nseg * kErrPartialsPerSegment));
-  c->seg_nnz = seg;
-  c->packed = packed;
-  c->stream_bytes = 4u * (N + 1);
-  if (packed) {
-    uint32_t most = 0;
-    for (int n = 0; n < N; ++n) {
-      uint32_t b = 4;
-      for (int k = 0; k < N; ++k)
-        if (k != n) b += (uint32_t)c->pack_bits[k] / 8;
-      most = std::max(most, b);
-    }
-    c->stream_bytes = most;
-  }
-  c->streaming = true;
+  setup_staging(c, fixed);
 }
 
 // The refined BF16 delta of the wide ranks (order 3, 16 <= R <= 32) runs.
@@ -1127,11 +1201,19 @@ void build_where(gt_ctx *c, int n) {
   GT_HIP(hipStreamSynchronize(s));
 }
 
+void do_partition_host(gt_ctx *c);
+
 void do_partition(gt_ctx *c) {
   require(c->have_coo, GT_ERR_STATE, "gt_set_coo first");
+  if (c->host_input) {
+    do_partition_host(c);
+    return;
+  }
   const int N = c->N;
   const uint64_t nnz = c->nnz;
   auto t0 = std::chrono::steady_clock::now();
+  c->host_partitioned = false;
+  c->partition_pieces = 0;
   uint64_t mem = c->o.device_budget;
   if (mem == 0) {
     size_t fr = 0, tot = 0;
@@ -1234,6 +1316,301 @@ void do_partition(gt_ctx *c) {
   c->partitioned = true;
   ensure_work(c);
   maybe_stream(c);
+  c->ms_partition =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  c->tm.ms_partition = c->ms_partition;
+}
+
+// The partition under a device budget (tensor_manager.tpp:201-302 and
+// block.tpp:114-186 with the data out of core): the input waits in pinned
+// host memory and is read in segments.  A first pass counts the nonzeros per
+// block and per row of every mode (the row pointers of the streams) and sums
+// the norm.  Then, for each mode n, consecutive ranges of rows ("buckets")
+// whose nonzeros fit in the device bytes left are built one at a time: a
+// pass over the input picks the bucket's nonzeros in input order, they are
+// sorted stably by block and then by row (the stream order: row, block,
+// position, exactly the in-core order), and the sorted piece goes to the
+// host stream at the row pointer of its first row, packed as in core (a
+// piece of a packed 16- or 24-bit stream must start at a multiple of four
+// nonzeros, so up to three trailing nonzeros wait for the next piece).  The
+// block COO is never resident; the ALS then stages the host streams as in
+// the out-of-core mode of an in-core partition.
+void do_partition_host(gt_ctx *c) {
+  const int N = c->N;
+  const uint64_t nnz = c->nnz, budget = c->o.device_budget;
+  auto t0 = std::chrono::steady_clock::now();
+  c->host_partitioned = true;
+  c->partition_pieces = 0;
+  Plan p;
+  if (!plan_partition(N, c->dims, nnz, c->Ru, budget, (int)sizeof(float), c->o.partition, p))
+    throw GtError(GT_ERR_PARTITION, "Block dimension is larger than the tensor dimension.");
+  require(p.blocks < (1ull << 32), GT_ERR_ARG, "too many blocks");
+  c->plan = p;
+  c->B = p.blocks;
+  for (int n = 0; n < N; ++n) c->rows[n] = (uint64_t)p.part[n] * p.bd[n];
+  hipStream_t s = c->s;
+  ensure_work(c);
+  const uint64_t fixed = resident_fixed_bytes(c);
+  if (fixed + (64ull << 10) >= budget)
+    throw GtError(GT_ERR_MEMORY, "device budget is below the per-row state");
+  const uint64_t avail = budget - fixed;
+  // device bytes per element: a scanned input segment (indices, value,
+  // flags, offsets, block keys and values) and a bucket (its nonzeros, the
+  // sort's four arrays and scratch, the sorted stream, the packing buffer)
+  const uint64_t per_scan = 4ull * N + 4 + 4 + 8 + 8;
+  const uint64_t per_bucket = (4ull * N + 4) + 16 + 8 + (4ull * N + 4) + 4;
+  uint64_t S = std::max<uint64_t>(std::min<uint64_t>(nnz, avail / 8 / per_scan), 1);
+  if (S < nnz) S = std::max<uint64_t>(S / 256 * 256, 256);
+  const uint64_t scan_bytes = S * per_scan + scan_temp_bytes(S + 1) + 4096;
+  require(scan_bytes < avail, GT_ERR_MEMORY, "device budget too small for a partition segment");
+  uint64_t cap = (avail - scan_bytes) / per_bucket;
+  while (cap > 1024 && cap * per_bucket + radix_sort_temp_bytes(cap) + 8192 > avail - scan_bytes)
+    cap -= cap / 16;
+  require(cap >= 1024, GT_ERR_MEMORY, "device budget too small for a partition bucket");
+
+  { // the partition's buffers are freed before the ALS staging is allocated
+  // ---- pass 1: block and row histograms, the norm
+  DevBuf<uint32_t> sidx[GT_MAX_ORDER], skeys, svals, flags, counts, rowcnt[GT_MAX_ORDER];
+  DevBuf<float> sval;
+  DevBuf<uint64_t> offs;
+  DevBuf<char> stemp;
+  for (int k = 0; k < N; ++k) sidx[k].alloc(S);
+  sval.alloc(S);
+  skeys.alloc(S);
+  svals.alloc(S);
+  flags.alloc(S);
+  offs.alloc(S + 1);
+  stemp.alloc(scan_temp_bytes(S + 1) + 4096);
+  counts.alloc(std::max<uint64_t>(c->B, 1));
+  fill_u32(counts.p, 0u, c->B, s);
+  for (int n = 0; n < N; ++n) {
+    rowcnt[n].alloc(c->rows[n]);
+    fill_u32(rowcnt[n].p, 0u, c->rows[n], s);
+  }
+  DevBuf<double> sq;
+  sq.alloc(1024);
+  std::vector<double> hsq(1024);
+  double ss = 0.0;
+  auto load_segment = [&](uint64_t lo, uint64_t len) {
+    for (int k = 0; k < N; ++k)
+      GT_HIP(hipMemcpyAsync(sidx[k].p, c->h_in_idx[k] + lo, len * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, s));
+    GT_HIP(hipMemcpyAsync(sval.p, c->h_in_val + lo, len * sizeof(float), hipMemcpyHostToDevice,
+                          s));
+  };
+  PartArgs pa{};
+  pa.order = N;
+  for (int k = 0; k < N; ++k) {
+    pa.bdims[k] = p.bd[k];
+    pa.part[k] = p.part[k];
+  }
+  for (uint64_t lo = 0; lo < nnz; lo += S) {
+    const uint64_t len = std::min(S, nnz - lo);
+    load_segment(lo, len);
+    pa.nnz = len;
+    for (int k = 0; k < N; ++k) pa.idx[k] = sidx[k].p;
+    block_ids(pa, (uint32_t)c->B, skeys.p, svals.p, counts.p, s);
+    for (int n = 0; n < N; ++n) histogram_u32(sidx[n].p, len, rowcnt[n].p, s);
+    const int g = sum_squares(sval.p, len, sq.p, s);
+    GT_HIP(hipMemcpyAsync(hsq.data(), sq.p, g * sizeof(double), hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < g; ++i) ss += hsq[i];
+  }
+  c->norm = std::sqrt(ss);
+  std::vector<uint32_t> hcounts(c->B);
+  GT_HIP(hipMemcpyAsync(hcounts.data(), counts.p, c->B * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        s));
+  std::vector<std::vector<uint64_t>> hrp(N);
+  for (int n = 0; n < N; ++n) {
+    c->row_ptr[n].alloc(c->rows[n] + 1);
+    DevBuf<char> t;
+    t.alloc(scan_temp_bytes(c->rows[n]) + 4096);
+    exclusive_scan_u32_to_u64(rowcnt[n].p, c->row_ptr[n].p, c->rows[n], t.p, s);
+    hrp[n].resize(c->rows[n] + 1);
+    GT_HIP(hipMemcpyAsync(hrp[n].data(), c->row_ptr[n].p, (c->rows[n] + 1) * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    rowcnt[n].reset();
+  }
+  c->block_nnz.assign(c->B, 0);
+  c->block_off.assign(c->B + 1, 0);
+  for (uint64_t b = 0; b < c->B; ++b) {
+    c->block_nnz[b] = hcounts[b];
+    c->block_off[b + 1] = c->block_off[b] + hcounts[b];
+  }
+
+  // ---- the host streams
+  const bool packed = choose_packing(c);
+  for (int n = 0; n < N; ++n) {
+    for (int k = 0; k < N; ++k) {
+      if (packed && k == n) continue;
+      if (packed)
+        GT_HIP(hipHostMalloc(&c->h_pack[n][k], pack_words(c->pack_bits[k], nnz) * sizeof(uint32_t),
+                             hipHostMallocDefault));
+      else
+        GT_HIP(hipHostMalloc(&c->h_idx[n][k], nnz * sizeof(uint32_t), hipHostMallocDefault));
+    }
+    GT_HIP(hipHostMalloc(&c->h_val[n], nnz * sizeof(float), hipHostMallocDefault));
+  }
+
+  // ---- pass 2: per mode, bucket by bucket
+  DevBuf<uint32_t> bidx[GT_MAX_ORDER], bkeys, bvals, btk, btv, sidx_out[GT_MAX_ORDER], ptmp,
+      carry[GT_MAX_ORDER];
+  DevBuf<float> bval, sval_out;
+  DevBuf<char> btemp;
+  for (int k = 0; k < N; ++k) {
+    bidx[k].alloc(cap);
+    sidx_out[k].alloc(cap + 4); // 4 leading slots for the carried nonzeros
+    carry[k].alloc(4);
+  }
+  bval.alloc(cap);
+  sval_out.alloc(cap);
+  bkeys.alloc(cap);
+  bvals.alloc(cap);
+  btk.alloc(cap);
+  btv.alloc(cap);
+  ptmp.alloc(3 * ((cap + 8) / 4) + 4);
+  btemp.alloc(radix_sort_temp_bytes(cap) + 4096);
+  DevBuf<uint32_t> bhist;
+  bhist.alloc(std::max<uint64_t>(c->B, 1));
+  std::vector<uint32_t> hbh(c->B);
+  for (int n = 0; n < N; ++n) {
+    const uint64_t rows = c->rows[n];
+    uint64_t cc = 0, out_off = 0; // carried nonzeros, offset of the next packed piece
+    uint64_t out_pos = 0;         // stream position of the next piece
+    // one piece: the nonzeros of rows [r0, r1) and blocks [b0, b1), and of
+    // those the ones ranked [w0, w1) in input order; `last` flushes the carry
+    auto piece = [&](uint64_t r0, uint64_t r1, uint32_t b0, uint32_t b1, bool all_blocks,
+                     uint64_t w0, uint64_t w1, uint64_t nb, bool last) {
+      if (nb > 0) {
+        uint64_t base = 0;
+        for (uint64_t lo = 0; lo < nnz; lo += S) {
+          const uint64_t len = std::min(S, nnz - lo);
+          load_segment(lo, len);
+          pa.nnz = len;
+          for (int k = 0; k < N; ++k) pa.idx[k] = sidx[k].p;
+          flag_piece(pa, n, (uint32_t)r0, (uint32_t)r1, b0, b1, all_blocks, flags.p, s);
+          exclusive_scan_u32_to_u64(flags.p, offs.p, len, stemp.p, s);
+          uint64_t cnt = 0;
+          GT_HIP(hipMemcpyAsync(&cnt, offs.p + len, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+          GT_HIP(hipStreamSynchronize(s));
+          if (cnt && base + cnt > w0 && base < w1) {
+            U32Ptrs src{};
+            U32MutPtrs dst{};
+            for (int k = 0; k < N; ++k) {
+              src.p[k] = sidx[k].p;
+              dst.p[k] = bidx[k].p;
+            }
+            scatter_flagged(N, len, flags.p, offs.p, base, w0, w1, src, sval.p, dst, bval.p, s);
+          }
+          base += cnt;
+          if (base >= w1) break; // the window is complete
+        }
+        require(std::min(base, w1) - w0 == nb, GT_ERR_HIP, "partition piece count mismatch");
+        // stable by block, then stable by row: the stream order
+        pa.nnz = nb;
+        for (int k = 0; k < N; ++k) pa.idx[k] = bidx[k].p;
+        block_ids(pa, (uint32_t)c->B, bkeys.p, bvals.p, counts.p, s);
+        radix_sort_pairs(bkeys.p, bvals.p, btk.p, btv.p, nb, ceil_log2(c->B), btemp.p, s);
+        U32Ptrs rsrc{};
+        U32MutPtrs rdst{};
+        rsrc.p[0] = bidx[n].p;
+        rdst.p[0] = bkeys.p;
+        gather_coo(1, nb, bvals.p, rsrc, nullptr, rdst, nullptr, s);
+        radix_sort_pairs(bkeys.p, bvals.p, btk.p, btv.p, nb, ceil_log2(rows), btemp.p, s);
+        U32Ptrs src{};
+        U32MutPtrs dst{};
+        for (int k = 0; k < N; ++k) {
+          src.p[k] = bidx[k].p;
+          dst.p[k] = sidx_out[k].p + 4;
+        }
+        gather_coo(N, nb, bvals.p, src, bval.p, dst, sval_out.p, s);
+        GT_HIP(hipMemcpyAsync(c->h_val[n] + out_pos, sval_out.p, nb * sizeof(float),
+                              hipMemcpyDeviceToHost, s));
+      }
+      if (!packed) {
+        for (int k = 0; k < N && nb > 0; ++k)
+          GT_HIP(hipMemcpyAsync(c->h_idx[n][k] + out_pos, sidx_out[k].p + 4,
+                                nb * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      } else if (nb > 0 || (last && cc > 0)) {
+        const uint64_t T = cc + nb, P = last ? T : T / 4 * 4;
+        for (int k = 0; k < N; ++k) {
+          if (k == n) continue;
+          uint32_t *work = sidx_out[k].p + 4 - cc;
+          if (cc)
+            GT_HIP(hipMemcpyAsync(work, carry[k].p, cc * sizeof(uint32_t),
+                                  hipMemcpyDeviceToDevice, s));
+          const int bits = c->pack_bits[k];
+          if (P) {
+            if (bits == 16)
+              pack16(work, P, ptmp.p, s);
+            else
+              pack24(work, P, ptmp.p, s);
+            GT_HIP(hipMemcpyAsync(c->h_pack[n][k] + pack_words(bits, out_off), ptmp.p,
+                                  pack_words(bits, P) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  s));
+          }
+          if (T > P)
+            GT_HIP(hipMemcpyAsync(carry[k].p, work + P, (T - P) * sizeof(uint32_t),
+                                  hipMemcpyDeviceToDevice, s));
+        }
+        out_off += P;
+        cc = T - P;
+      }
+      GT_HIP(hipStreamSynchronize(s)); // the buffers are reused by the next piece
+      out_pos += nb;
+      ++c->partition_pieces;
+    };
+    uint64_t r0 = 0;
+    while (r0 < rows) {
+      const uint64_t rc = hrp[n][r0 + 1] - hrp[n][r0];
+      if (rc <= cap) { // whole rows: as many as fit
+        uint64_t r1 = r0 + 1;
+        while (r1 < rows && hrp[n][r1 + 1] - hrp[n][r0] <= cap) ++r1;
+        piece(r0, r1, 0, (uint32_t)c->B, true, 0, ~0ull, hrp[n][r1] - hrp[n][r0], r1 == rows);
+        r0 = r1;
+        continue;
+      }
+      // a row with more nonzeros than a piece holds: by block ranges, and a
+      // block with more by windows of its input order (its stream order)
+      fill_u32(bhist.p, 0u, c->B, s);
+      for (uint64_t lo = 0; lo < nnz; lo += S) {
+        const uint64_t len = std::min(S, nnz - lo);
+        load_segment(lo, len);
+        pa.nnz = len;
+        for (int k = 0; k < N; ++k) pa.idx[k] = sidx[k].p;
+        row_block_hist(pa, n, (uint32_t)r0, bhist.p, s);
+      }
+      GT_HIP(hipMemcpyAsync(hbh.data(), bhist.p, c->B * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                            s));
+      GT_HIP(hipStreamSynchronize(s));
+      const bool last_row = r0 + 1 == rows;
+      uint64_t b = 0;
+      while (b < c->B) {
+        if (hbh[b] > cap) {
+          for (uint64_t w = 0; w < hbh[b]; w += cap) {
+            const uint64_t w1 = std::min<uint64_t>(w + cap, hbh[b]);
+            piece(r0, r0 + 1, (uint32_t)b, (uint32_t)b + 1, false, w, w1, w1 - w,
+                  last_row && b + 1 == c->B && w1 == hbh[b]);
+          }
+          ++b;
+          continue;
+        }
+        uint64_t b1 = b + 1, sum = hbh[b];
+        while (b1 < c->B && sum + hbh[b1] <= cap) sum += hbh[b1++];
+        piece(r0, r0 + 1, (uint32_t)b, (uint32_t)b1, false, 0, ~0ull, sum, last_row && b1 == c->B);
+        b = b1;
+      }
+      ++r0;
+    }
+    require(cc == 0, GT_ERR_HIP, "partition left nonzeros unpacked");
+    require(out_pos == nnz, GT_ERR_HIP, "partition stream length mismatch");
+  }
+  } // the partition's device buffers go here
+  c->release_host_input();
+  c->have_coo = false;
+  c->partitioned = true;
+  setup_staging(c, fixed);
   c->ms_partition =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   c->tm.ms_partition = c->ms_partition;
@@ -1413,15 +1790,28 @@ int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const floa
     for (int k = 0; k < ctx->N; ++k) require(dims[k] > 0, GT_ERR_ARG, "zero dimension");
     for (int k = 0; k < ctx->N; ++k) require(idx[k] != nullptr || nnz == 0, GT_ERR_ARG, "null idx");
     ctx->release_host_streams();
+    ctx->release_host_input();
     ctx->drop_graph();
     ctx->partitioned = false;
     ctx->initialized = false;
     ctx->nnz = nnz;
+    // under a device budget the in-core partition does not fit, the input
+    // stays in pinned host memory for the bucketed partition
+    ctx->host_input = ctx->o.device_budget != 0 &&
+                      nnz * incore_partition_bytes_per_nnz(ctx->N) + fixed_estimate(ctx, nnz, dims) >
+                          ctx->o.device_budget;
     for (int k = 0; k < ctx->N; ++k) {
       ctx->dims[k] = dims[k];
       // bounds check on the host copy (tensor_manager.tpp:237 asserts it)
       for (uint64_t e = 0; e < nnz; ++e)
         if (idx[k][e] >= dims[k]) throw GtError(GT_ERR_ARG, "Coordinate is out of bounds");
+      if (ctx->host_input) {
+        ctx->in_idx[k].reset();
+        GT_HIP(hipHostMalloc(&ctx->h_in_idx[k], std::max<uint64_t>(nnz, 1) * sizeof(uint32_t),
+                             hipHostMallocDefault));
+        std::memcpy(ctx->h_in_idx[k], idx[k], nnz * sizeof(uint32_t));
+        continue;
+      }
       ctx->in_idx[k].alloc(nnz);
       if (nnz)
         GT_HIP(hipMemcpyAsync(ctx->in_idx[k].p, idx[k], nnz * sizeof(uint32_t),
@@ -1432,10 +1822,17 @@ int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const floa
       for (uint64_t e = 0; e < nnz; ++e) ss += vals[e] * vals[e];
       ctx->norm_ref = std::sqrt(ss);
     }
-    ctx->in_val.alloc(nnz);
-    if (nnz)
-      GT_HIP(hipMemcpyAsync(ctx->in_val.p, vals, nnz * sizeof(float), hipMemcpyHostToDevice,
-                            ctx->s));
+    if (ctx->host_input) {
+      ctx->in_val.reset();
+      GT_HIP(hipHostMalloc(&ctx->h_in_val, std::max<uint64_t>(nnz, 1) * sizeof(float),
+                           hipHostMallocDefault));
+      std::memcpy(ctx->h_in_val, vals, nnz * sizeof(float));
+    } else {
+      ctx->in_val.alloc(nnz);
+      if (nnz)
+        GT_HIP(hipMemcpyAsync(ctx->in_val.p, vals, nnz * sizeof(float), hipMemcpyHostToDevice,
+                              ctx->s));
+    }
     GT_HIP(hipStreamSynchronize(ctx->s));
     ctx->have_coo = true;
   });
@@ -1477,6 +1874,8 @@ int gt_get_partition_info(gt_ctx *ctx, gt_partition_info *info) {
     info->out_of_core = ctx->streaming ? 1 : 0;
     info->segment_nnz = ctx->seg_nnz;
     info->stream_bytes_per_nnz = ctx->streaming ? ctx->stream_bytes : 0;
+    info->partition_path = ctx->host_partitioned ? 1 : 0;
+    info->partition_pieces = ctx->partition_pieces;
   });
 }
 

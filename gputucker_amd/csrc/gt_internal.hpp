@@ -111,6 +111,18 @@ struct U32MutPtrs {
 // (dst_val may be null).
 void gather_coo(int order, uint64_t nnz, const uint32_t *perm, U32Ptrs src, const float *src_val,
                 U32MutPtrs dst, float *dst_val, hipStream_t s);
+// flags[e] = 1 for the nonzeros of a.idx (a.nnz of them) whose row of `mode`
+// is in [r0, r1) and, unless all_blocks, whose block is in [b0, b1).
+void flag_piece(const PartArgs &a, int mode, uint32_t r0, uint32_t r1, uint32_t b0, uint32_t b1,
+                bool all_blocks, uint32_t *flags, hipStream_t s);
+// hist[block] += 1 for each nonzero of a.idx whose row of `mode` is r.
+void row_block_hist(const PartArgs &a, int mode, uint32_t r, uint32_t *hist, hipStream_t s);
+// Stable compaction with a window: each flagged e has rank base + offs[e]
+// (offs the exclusive scan of flags); those with rank in [w0, w1) go to
+// dst.p[k][rank - w0] (and the value).
+void scatter_flagged(int order, uint64_t n, const uint32_t *flags, const uint64_t *offs,
+                     uint64_t base, uint64_t w0, uint64_t w1, U32Ptrs src, const float *src_val,
+                     U32MutPtrs dst, float *dst_val, hipStream_t s);
 // counts[key[e]]++ for e < n (counts must be zeroed).
 void histogram_u32(const uint32_t *key, uint64_t n, uint32_t *counts, hipStream_t s);
 // Per-block partial sums of x^2 in FP64 (deterministic order); returns the

@@ -70,6 +70,56 @@ __global__ void k_gather_coo(int order, uint64_t nnz, const uint32_t *__restrict
   }
 }
 
+__device__ inline uint32_t block_id_of(const PartArgs &a, uint64_t e) {
+  uint32_t id = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxOrder; ++k)
+    if (k < a.order) id = id * a.part[k] + a.idx[k][e] / a.bdims[k];
+  return id;
+}
+
+// flags[e] = 1 for the nonzeros of a partition piece: row of `mode` in
+// [r0, r1) and block in [b0, b1) (the block test skipped when it is every block)
+__global__ void k_flag_piece(PartArgs a, int mode, uint32_t r0, uint32_t r1, uint32_t b0,
+                             uint32_t b1, bool all_blocks, uint32_t *__restrict__ flags) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nnz;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t v = a.idx[mode][e];
+    bool in = v >= r0 && v < r1;
+    if (in && !all_blocks) {
+      const uint32_t b = block_id_of(a, e);
+      in = b >= b0 && b < b1;
+    }
+    flags[e] = in ? 1u : 0u;
+  }
+}
+
+// hist[block] += 1 for the nonzeros whose row of `mode` is r
+__global__ void k_row_block_hist(PartArgs a, int mode, uint32_t r, uint32_t *__restrict__ hist) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nnz;
+       e += (uint64_t)gridDim.x * blockDim.x)
+    if (a.idx[mode][e] == r) atomicAdd(&hist[block_id_of(a, e)], 1u);
+}
+
+// The flagged elements whose rank (base + offs[e], offs the exclusive scan of
+// flags) lies in [w0, w1) go to dst[rank - w0], in order.
+__global__ void k_scatter_flagged(int order, uint64_t n, const uint32_t *__restrict__ flags,
+                                  const uint64_t *__restrict__ offs, uint64_t base, uint64_t w0,
+                                  uint64_t w1, U32Ptrs src, const float *__restrict__ src_val,
+                                  U32MutPtrs dst, float *__restrict__ dst_val) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    if (!flags[e]) continue;
+    const uint64_t rank = base + offs[e];
+    if (rank < w0 || rank >= w1) continue;
+    const uint64_t j = rank - w0;
+#pragma unroll
+    for (int k = 0; k < kMaxOrder; ++k)
+      if (k < order) dst.p[k][j] = src.p[k][e];
+    dst_val[j] = src_val[e];
+  }
+}
+
 __global__ void k_histogram(const uint32_t *__restrict__ key, uint64_t n,
                             uint32_t *__restrict__ counts) {
   for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
@@ -217,6 +267,30 @@ void gather_coo(int order, uint64_t nnz, const uint32_t *perm, U32Ptrs src, cons
   if (!nnz) return;
   hipLaunchKernelGGL(k_gather_coo, dim3(grid_for(nnz, 256)), dim3(256), 0, s, order, nnz, perm,
                      src, src_val, dst, dst_val);
+  GT_HIP(hipGetLastError());
+}
+
+void flag_piece(const PartArgs &a, int mode, uint32_t r0, uint32_t r1, uint32_t b0, uint32_t b1,
+                bool all_blocks, uint32_t *flags, hipStream_t s) {
+  if (!a.nnz) return;
+  hipLaunchKernelGGL(k_flag_piece, dim3(grid_for(a.nnz, 256)), dim3(256), 0, s, a, mode, r0, r1,
+                     b0, b1, all_blocks, flags);
+  GT_HIP(hipGetLastError());
+}
+
+void row_block_hist(const PartArgs &a, int mode, uint32_t r, uint32_t *hist, hipStream_t s) {
+  if (!a.nnz) return;
+  hipLaunchKernelGGL(k_row_block_hist, dim3(grid_for(a.nnz, 256)), dim3(256), 0, s, a, mode, r,
+                     hist);
+  GT_HIP(hipGetLastError());
+}
+
+void scatter_flagged(int order, uint64_t n, const uint32_t *flags, const uint64_t *offs,
+                     uint64_t base, uint64_t w0, uint64_t w1, U32Ptrs src, const float *src_val,
+                     U32MutPtrs dst, float *dst_val, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_scatter_flagged, dim3(grid_for(n, 256)), dim3(256), 0, s, order, n, flags,
+                     offs, base, w0, w1, src, src_val, dst, dst_val);
   GT_HIP(hipGetLastError());
 }
 

@@ -32,12 +32,17 @@ def generate(dims, nnz, modes=None, values="uniform", seed=0, device=None):
         total *= d
     if nnz > total:
         raise ValueError("more nonzeros than cells")
-    if total >= 2.0 ** 63:
-        raise ValueError("index space too large for 64-bit keys")
     strides = [1] * N
     for k in range(N - 2, -1, -1):
         strides[k] = strides[k + 1] * dims[k + 1]
     use_torch = device is not None and str(device).startswith("cuda")
+    if total >= 2.0 ** 62:
+        # two 62-bit key words: the leading modes and the trailing ones
+        split = next(j for j in range(1, N) if float(np.prod([float(d) for d in dims[j:]])) < 2.0 ** 62
+                     and float(np.prod([float(d) for d in dims[:j]])) < 2.0 ** 62)
+        if not use_torch:
+            raise ValueError("index spaces beyond 2^62 cells are generated on the GPU")
+        return _generate_torch_wide(dims, nnz, modes, values, seed, device, split)
     if use_torch:
         return _generate_torch(dims, nnz, modes, values, seed, device, strides)
     return _generate_numpy(dims, nnz, modes, values, seed, strides)
@@ -99,6 +104,70 @@ def _generate_torch(dims, nnz, modes, values, seed, device, strides):
         v = torch.exp(torch.randn(nnz, generator=g, device=device, dtype=torch.float32))
     vals = v.cpu().numpy()
     del keys, v
+    torch.cuda.empty_cache()
+    return idx, vals, dims
+
+
+def _generate_torch_wide(dims, nnz, modes, values, seed, device, split):
+    """As _generate_torch, for index spaces of 2^62 cells and more: a
+    coordinate is two key words (modes [0, split) and [split, N)), and
+    duplicates are found after a lexicographic sort of the word pairs (stable
+    sort by the second word, then by the first)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    N = len(dims)
+    cdfs = [None if m == "uniform" else torch.from_numpy(_zipf_cdf(dims[k], m[1])).to(device)
+            for k, m in enumerate(modes)]
+    groups = [list(range(split)), list(range(split, N))]
+    gstr = []
+    for grp in groups:
+        st = {}
+        mult = 1
+        for k in reversed(grp):
+            st[k] = mult
+            mult *= dims[k]
+        gstr.append(st)
+    hi = torch.zeros(0, dtype=torch.int64, device=device)
+    lo = torch.zeros(0, dtype=torch.int64, device=device)
+    while hi.numel() < nnz:
+        m = int((nnz - hi.numel()) * 1.25) + 1024
+        words = []
+        for grp, st in zip(groups, gstr):
+            w = torch.zeros(m, dtype=torch.int64, device=device)
+            for k in grp:
+                if cdfs[k] is None:
+                    col = torch.randint(0, dims[k], (m,), generator=g, device=device)
+                else:
+                    u = torch.rand(m, generator=g, device=device, dtype=torch.float64)
+                    col = torch.searchsorted(cdfs[k], u, right=True).clamp_(max=dims[k] - 1)
+                w += col.to(torch.int64) * st[k]
+                del col
+            words.append(w)
+        hi = torch.cat([hi, words[0]])
+        lo = torch.cat([lo, words[1]])
+        del words
+        o = torch.sort(lo, stable=True).indices
+        o = o[torch.sort(hi[o], stable=True).indices]
+        hi, lo = hi[o], lo[o]
+        del o
+        keep = torch.ones(hi.numel(), dtype=torch.bool, device=device)
+        keep[1:] = (hi[1:] != hi[:-1]) | (lo[1:] != lo[:-1])
+        hi, lo = hi[keep], lo[keep]
+        del keep
+    perm = torch.randperm(hi.numel(), generator=g, device=device)[:nnz]
+    hi, lo = hi[perm], lo[perm]
+    del perm
+    idx = np.zeros((N, nnz), np.uint32)
+    for w, grp, st in ((hi, groups[0], gstr[0]), (lo, groups[1], gstr[1])):
+        for k in grp:
+            idx[k] = ((w // st[k]) % dims[k]).to(torch.int32).cpu().numpy().view(np.uint32)
+    if values == "uniform":
+        v = torch.rand(nnz, generator=g, device=device, dtype=torch.float32)
+    else:
+        v = torch.exp(torch.randn(nnz, generator=g, device=device, dtype=torch.float32))
+    vals = v.cpu().numpy()
+    del hi, lo, v
     torch.cuda.empty_cache()
     return idx, vals, dims
 

@@ -110,6 +110,11 @@ typedef struct gt_partition_info {
                                     the largest over the modes (indices packed to 16 or 24 bits
                                     and no row index when every factor has at most 2^24 rows,
                                     else 4 bytes per index) */
+  int partition_path;        /* 0: partitioned in HBM; 1: the in-core partition exceeded the device
+                                budget, so the input waited in pinned host memory and each mode's
+                                stream was built piece by piece (ranges of rows, of blocks inside
+                                a heavy row, windows inside a heavy block) under the budget */
+  uint64_t partition_pieces; /* pieces the bucketed partition built, over all modes */
 } gt_partition_info;
 
 typedef struct gt_timings {

@@ -374,12 +374,15 @@ def test_out_of_core_matches_in_core(prec, pack, monkeypatch):
     ref = make(idx, vals, dims, 6, [2, 1, 2], **kw)
     ref.set_init(A, G)
     assert ref.info()["out_of_core"] == 0
-    # about 1.1 MB of per-row state stays resident; 1.5 MB leaves room for a
-    # few thousand nonzeros per staged segment
+    # about 1.4 MB of per-row state stays resident; 1.5 MB leaves room for a
+    # few thousand nonzeros per staged segment, and the partition itself runs
+    # from host memory in pieces of about 1500 nonzeros: the Zipf rows are
+    # split by block and the heaviest blocks by windows of their input order
     oc = make(idx, vals, dims, 6, [2, 1, 2], device_budget=1_500_000, **kw)
     oc.set_init(A, G)
     info = oc.info()
     assert info["out_of_core"] == 1
+    assert info["partition_path"] == 1 and info["partition_pieces"] > 3 * 10, info
     assert info["stream_bytes_per_nnz"] == (8 if pack else 16)
     assert 0 < info["segment_nnz"] < 30000 and info["segment_nnz"] % 64 == 0
     f_ref, f_oc = ref.run(), oc.run()
@@ -490,12 +493,16 @@ def test_out_of_core_24bit_indices(prec):
     kw = dict(precision=prec, chunk_tiles=1, max_iters=2, tol=0.0)
     ref = make(idx, vals, dims, 4, [1, 1, 1], **kw)
     ref.set_init(A, G)
-    # per-row state (Gram rows, factors, row pointers) is about 10.6 MB here;
-    # 11.2 MB leaves room for two segments of about 11000 nonzeros
-    oc = make(idx, vals, dims, 4, [1, 1, 1], device_budget=11_200_000, **kw)
+    # per-row state (Gram rows, factors and for BF16 their split copies, row
+    # pointers) is about 10.6 MB here (15.1 MB for bf16r); 0.56 MB more leaves
+    # room for two segments of about 11000 nonzeros, and for partition pieces
+    # of about 8000, which start at offsets that are not multiples of four
+    budget = 11_200_000 if prec == "fp32" else 15_700_000
+    oc = make(idx, vals, dims, 4, [1, 1, 1], device_budget=budget, **kw)
     oc.set_init(A, G)
     info = oc.info()
     assert info["out_of_core"] == 1 and ref.info()["out_of_core"] == 0
+    assert info["partition_path"] == 1 and info["partition_pieces"] >= 3 * 2, info
     assert info["stream_bytes_per_nnz"] == 9
     assert -(-20000 // info["segment_nnz"]) >= 2
     f_ref, f_oc = ref.run(), oc.run()
@@ -590,3 +597,41 @@ def test_autotune_keeps_parity(prec, N, R, part):
     assert np.allclose(fits, fits_ref, rtol=1e-7, atol=0), (fits, fits_ref)
     f64, _ = oracle.als(t, A, G, 1e-3, 3, 0.0, "f64")
     assert (np.abs(fits - f64) / np.abs(f64)).max() < (1e-5 if prec == "fp32" else 1e-4)
+
+
+def test_host_partition_windows_inside_a_heavy_block():
+    """A row holding most of the tensor in one block (P = 1): under a tight
+    device budget the bucketed partition cuts it into windows of its input
+    order, a few thousand nonzeros each; the streams, and so the factors,
+    are those of the in-core partition, bit for bit.  Also the
+    order-5 case with 16-bit packing over many pieces."""
+    from gputucker_amd import synth
+    print("host partition: heavy row windows", flush=True)
+    idx, vals, dims = synth.generate([60, 500, 400], 20000, [("zipf", 2.0), "uniform", "uniform"],
+                                     "uniform", seed=4)
+    A, G = synth.init_factors(dims, 5, seed=4)
+    kw = dict(precision="bf16r", chunk_tiles=1, max_iters=2, tol=0.0)
+    ref = make(idx, vals, dims, 5, [1, 1, 1], **kw)
+    ref.set_init(A, G)
+    oc = make(idx, vals, dims, 5, [1, 1, 1], device_budget=1_600_000, **kw)
+    oc.set_init(A, G)
+    info = oc.info()
+    assert info["partition_path"] == 1 and info["out_of_core"] == 1, info
+    assert np.bincount(idx[0]).max() > 8000  # the heavy row: about 12000 of the 20000
+    assert np.array_equal(ref.block_nnz, oc.block_nnz)
+    assert abs(ref.info()["norm"] - info["norm"]) <= 1e-12 * info["norm"]
+    f_ref, f_oc = ref.run(), oc.run()
+    for m in range(3):
+        assert np.array_equal(ref.get_factor(m), oc.get_factor(m)), m
+    assert np.allclose(f_ref, f_oc, rtol=0, atol=1e-12)
+    idx, vals, dims = synth.generate([300, 300, 300, 300, 20], 40_000, None, "uniform", seed=8)
+    A, G = synth.init_factors(dims, 4, seed=8)
+    kw = dict(precision="fp32", chunk_tiles=1, max_iters=1, tol=0.0)
+    ref = make(idx, vals, dims, 4, [1, 2, 1, 1, 1], **kw)
+    ref.set_init(A, G)
+    oc = make(idx, vals, dims, 4, [1, 2, 1, 1, 1], device_budget=1_600_000, **kw)
+    oc.set_init(A, G)
+    assert oc.info()["partition_path"] == 1 and oc.info()["stream_bytes_per_nnz"] == 4 * 2 + 4
+    f_ref, f_oc = ref.run(), oc.run()
+    for m in range(5):
+        assert np.array_equal(ref.get_factor(m), oc.get_factor(m)), m

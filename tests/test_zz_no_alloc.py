@@ -43,7 +43,7 @@ def test_fit_noalloc_matches_fit_on_consecutive_calls():
 @pytest.mark.parametrize("budget,fits_scratch", [(3_000_000, True), (1_500_000, False)])
 def test_fit_noalloc_out_of_core(budget, fits_scratch):
     """Out of core the pass writes 2048 partials per staged segment: a layout
-    of two segments fits the scratch, one of eleven is refused."""
+    of four segments fits the scratch, one of eleven is refused."""
     print(f"noalloc: out of core, budget {budget}", flush=True)
     kw = dict(chunk_tiles=1, lam=1e-3)
     e = _engine(11, [300, 200, 100], 30000, 6, [2, 1, 2], device_budget=budget, **kw)
