@@ -11,11 +11,11 @@ of (iteration time without the fit) / N, and value = nnz / t_mode.  The fit
 is computed after the timed region.
 
     python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5s] [--precision fp32]
-                    [--no-graph] [--no-autotune] [--budget BYTES] [--no-cpu]
+                    [--no-graph] [--autotune] [--budget BYTES] [--no-cpu]
 
 By default each ALS iteration replays a captured HIP graph (the N mode
-updates and the fit reduction) and the chunk size of each mode is picked by
-the engine's autotuner at initialisation.
+updates and the fit reduction); --autotune lets the engine's autotuner pick
+each mode's chunk size at initialisation.
 """
 from __future__ import annotations
 
@@ -123,6 +123,23 @@ def kernel_bytes_per_nnz(N, R, path):
 
 def flops_per_nnz(N, R):
     return 2 * R ** N + 2 * R * R + 2 * R
+
+
+def limiter(ctr):
+    """What the committed counters say bounds the kernel: HBM when its traffic
+    runs near the peak; the SIMDs' issue when MFMA busy plus VALU busy (which
+    do not overlap on gfx950) fills most cycles; else latency (the waves
+    waiting on memory and LDS with neither pipe nor HBM saturated)."""
+    if not ctr or ctr.get("mfma_busy") is None:
+        return None
+    issue = ctr["mfma_busy"] + (ctr.get("valu_busy") or 0.0)
+    hbm = ctr.get("hbm_util") or 0.0
+    if hbm >= 0.75:
+        return f"hbm ({hbm:.0%} of 8 TB/s)"
+    if issue >= 0.75:
+        return f"issue (MFMA {ctr['mfma_busy']:.0%} + VALU {ctr.get('valu_busy') or 0:.0%})"
+    return (f"latency (issue {issue:.0%}: MFMA {ctr['mfma_busy']:.0%} + VALU "
+            f"{ctr.get('valu_busy') or 0:.0%}; HBM {hbm:.0%})")
 
 
 def cpu_model():
@@ -247,8 +264,9 @@ def main(argv=None):
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="launch each iteration's kernels eagerly instead of replaying the "
                          "captured HIP graph")
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="keep the default chunk size instead of timing the candidates")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick each mode's chunk size (and core-operand source) by timing the "
+                         "candidates at initialisation")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is "
@@ -339,7 +357,8 @@ def main(argv=None):
                      "mfma_util": ctr.get("mfma_busy") if ctr else None,
                      "valu_util": ctr.get("valu_busy") if ctr else None,
                      "hbm_util": ctr.get("hbm_util") if ctr else None,
-                     "limiter": ctr.get("limiter") if ctr else None},
+                     "limiter": limiter(ctr),
+                     "counters": f"profiles/counters_{a.config}.md" if ctr else None},
         "cpu_baseline": cpu,
     }
     if head["out_of_core"]:  # the host-link rate of the staged streams (never `value`)
