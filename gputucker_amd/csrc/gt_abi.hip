@@ -733,7 +733,7 @@ GramArgs gram_args(gt_ctx *c, int n) {
   a.nchunks = (c->nnz + a.chunk - 1) / a.chunk;
   a.chunk_begin = 0;
   a.chunk_end = a.nchunks;
-  a.prefer_l2 = c->streaming ? 0 : c->mvariant[n];
+  a.alt_source = c->streaming ? 0 : c->mvariant[n];
   a.rows = c->rows[n];
   a.Gb = c->Gb[n].p;
   // slot order: L = last mode other than n, outer slots = the rest ascending
@@ -850,8 +850,8 @@ void run_gram(gt_ctx *c, int n) {
 
 // The project's autotuner (in core): for each mode, times the delta + Gram
 // pass (with its fix-up) at chunk sizes of 1 to 32 tiles and, where both
-// kernels apply (refined BF16 at orders 4-5 with a core table that fits in
-// LDS), with the core operand from LDS and from L2; keeps the fastest.  The
+// sources of the core operand apply (refined BF16: a table in LDS or L2),
+// with each; keeps the fastest.  The
 // passes only overwrite the Gram scratch, not the factors.  The chunking
 // changes the order of the FP64 sums inside a row, so results move at FP64
 // rounding.
@@ -872,7 +872,7 @@ void autotune(gt_ctx *c) {
     uint64_t best_chunk = c->chunk;
     int best_var = 0;
     for (int var = 0; var < 2; ++var) {
-      if (var == 1) { // the L2 variant: only where it differs from the default
+      if (var == 1) { // the other core-operand source: only where it is another kernel
         c->mvariant[n] = 1;
         GramArgs probe = gram_args(c, n);
         probe.chunk_end = 0; // pick the kernel, launch nothing

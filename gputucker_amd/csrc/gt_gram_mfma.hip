@@ -588,37 +588,67 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
 
 // ------------------------------------------------------------ BF16, orders 4-5
 // The refined BF16 delta of an order-4 or order-5 tensor (NS = 2 or 3 outer
-// slots) whose core table does not fit in LDS beside the tiles (order 4,
-// R = 8: 64 outer indices, a 64 KB table; order 5 needs R <= 8).  The
-// core operand of each MFMA comes from the table in global memory (it stays
-// in L2), two outer indices ahead; s_m[o] = prod_t A_t[i_t][o_t] is formed at
-// its use from the outer rows staged in LDS.  The next tile's gathers
-// (split A_L words and the outer rows) are issued after this tile's stage
-// loop, so the core loads never wait behind them.
-template <int R, int NS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+// slots) whose per-wave s table does not leave the order-3 kernel enough
+// waves (order 4, R = 8: 64 outer indices; order 5 needs R <= 8).  The core
+// operand of each MFMA comes either from the table in global memory (it stays
+// in L2), kPairRing steps ahead, or (TL, pair table only) from a copy of the
+// pair table in LDS that the workgroup's waves share; s_m[o] =
+// prod_t A_t[i_t][o_t] is formed at its use from the outer rows staged in
+// LDS.  The next tile's gathers (split A_L words and the outer rows) are
+// issued after this tile's stage loop, so the core loads never wait behind
+// them.
+// core operands in flight in the pair loop (a ring of registers, one per step)
+constexpr int kPairRing = 2;
+// lanes of the pair table that hold words: lane group 3 is zero for R <= 8
+// (bf16r_plan), so the LDS copy keeps lanes 0..47 of each pair
+constexpr int kPairLanes = 48;
+// largest workgroup of the LDS-table variant: 12 waves, three per SIMD (the
+// VGPR budget of amdgpu_waves_per_eu(3))
+constexpr int kO4MaxWaves = 12;
+
+// LDS floats per wave of k_gram_bf16_o4: the outer rows, the delta tile, the
+// tile's row ids
+template <int R, int NS> constexpr size_t o4_lds_floats_per_wave() {
+  return (size_t)NS * R * kWave + 16 * kDP + kWave;
+}
+// LDS bytes of the pair table's copy (TL variant)
+template <int R, int NS> constexpr size_t o4_table_bytes() {
+  return (size_t)gb_pairs(cpow(R, NS)) * kPairLanes * 16;
+}
+
+template <int R, int NS, bool TL>
+__global__ void __launch_bounds__(64 * kO4MaxWaves) __attribute__((amdgpu_waves_per_eu(3)))
 k_gram_bf16_o4(GramArgs a, int wpb) {
   static_assert(bf16r_single_mfma(R) && R >= 2, "one refined MFMA per block and o");
   static_assert(NS == 2 || (NS == 3 && bf16r_pair_table(R)), "orders 4 and 5 (R <= 8)");
+  static_assert(!TL || bf16r_pair_table(R), "the LDS table variant takes the pair table");
   constexpr int RS = row_stride(R);
   constexpr int T = ChunkGram<R>::T;
   constexpr int O = cpow(R, NS);
-  __shared__ float s_tile[4][16 * kDP];
-  __shared__ uint32_t s_row[4][kWave];
   extern __shared__ __align__(16) float dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
-  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
-  if (c >= a.chunk_end) return;
-  float *lrows = dyn + (size_t)w * NS * R * kWave; // [slot][j][nonzero]
-  float *tile = s_tile[w];
-  const uint64_t cb = c * a.chunk;
-  const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
-  const int q = lane & 15, g = lane >> 4;
   // R <= 8: the pair table, one MFMA per two outer indices; else the K32 table
   constexpr bool kPair = bf16r_pair_table(R);
   constexpr int NB = kPair ? gb_pairs(O) : O; // MFMAs per 16 nonzeros
   const s8 *tg = reinterpret_cast<const s8 *>(
       static_cast<const uint16_t *>(a.Gb) + gb_k16_words(O) + (kPair ? 2 * gb_k32_words(O) : 0));
+  constexpr size_t kTabFloats = TL ? o4_table_bytes<R, NS>() / sizeof(float) : 0;
+  const s8 *tl = reinterpret_cast<const s8 *>(dyn);
+  if constexpr (TL) { // the pair table's nonzero lanes to LDS, once per workgroup
+    uint4 *dst = reinterpret_cast<uint4 *>(dyn);
+    const uint4 *src = reinterpret_cast<const uint4 *>(tg);
+    for (int i = threadIdx.x; i < NB * kPairLanes; i += blockDim.x)
+      dst[i] = src[(i / kPairLanes) * kWave + i % kPairLanes];
+    __syncthreads();
+  }
+  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
+  if (c >= a.chunk_end) return;
+  float *lrows = dyn + kTabFloats + (size_t)w * o4_lds_floats_per_wave<R, NS>(); // [slot][j][nonzero]
+  float *tile = lrows + NS * R * kWave;
+  uint32_t *s_rowv = reinterpret_cast<uint32_t *>(tile + 16 * kDP);
+  const uint64_t cb = c * a.chunk;
+  const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
+  const int q = lane & 15, g = lane >> 4;
   ChunkGram<R> gram(a, c, cb, ce, lane);
 
   uint32_t n_row, n_iL, n_io[NS];
@@ -658,7 +688,7 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
     const float x = c_x;
     c_row = n_row;
     c_x = n_x;
-    s_row[w][lane] = valid ? row : 0xFFFFFFFFu;
+    s_rowv[lane] = valid ? row : 0xFFFFFFFFu;
     // this tile's outer rows to LDS, before the next gather reuses them
 #pragma unroll
     for (int t = 0; t < NS; ++t)
@@ -670,26 +700,35 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
-    s8 bA = tg[lane], bB = tg[kWave + lane];
     if constexpr (kPair) {
       // MFMA p covers outer indices 2p (output rows 0..7) and 2p + 1 (rows
       // 8..15): lane (q, g) gets U[16mb + q][o][r] with o = 2p + g / 2 and
-      // r = 4 (g % 2) + i, so it scales by s_m[o] of its own o
-      auto pair_step = [&](int p, int nb) {
-        const b8 b = __builtin_bit_cast(b8, bA);
-        bA = bB;
-        if (p + 2 < nb) bB = tg[(p + 2) * kWave + lane];
+      // r = 4 (g % 2) + i, so it scales by s_m[o] of its own o.  The digits
+      // of both outer indices of a step are wave-uniform (scalar work); a
+      // lane only selects its half's.  The core operands come from L2
+      // kPairRing steps ahead, through a ring of registers.
+      const bool odd = g >= 2;
+      auto pair_step = [&](int p, const s8 &bw) {
+        const b8 b = __builtin_bit_cast(b8, bw);
         f4 u[4];
         const f4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb)
           u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, g32[mb]), z, 0,
                                                           0, 0);
-        const int o = min(2 * p + (g >> 1), O - 1); // past O the core words are zero
+        // past O the core words are zero: any in-range outer index will do
+        const int oe = min(2 * p, O - 1), oo = min(2 * p + 1, O - 1);
         const float *l[NS];
 #pragma unroll
-        for (int t = 0; t < NS; ++t)
-          l[t] = lrows + (t * R + o / cpow(R, NS - 1 - t) % R) * kWave + q;
+        for (int t = 0; t < NS; ++t) {
+          // both offsets are computed in scalar registers and hidden from the
+          // optimiser: otherwise it folds the select into the digits of a
+          // per-lane 2p + half, with vector divisions by R at every step
+          int offe = (t * R + oe / cpow(R, NS - 1 - t) % R) * kWave;
+          int offo = (t * R + oo / cpow(R, NS - 1 - t) % R) * kWave;
+          asm volatile("" : "+s"(offe), "+s"(offo));
+          l[t] = lrows + q + (odd ? offo : offe);
+        }
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
           float sv = l[0][mb * 16];
@@ -699,15 +738,38 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
           for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
         }
       };
-      static_assert(NB % 2 == 0, "the pair table has an even number of pairs");
+      if constexpr (TL) {
+        // lane group 3 takes zeros (its words are zero in the table too)
+        const s8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+        const int tl_lane = g < 3 ? lane : 0;
 #pragma unroll 2
-      for (int p = 0; p < NB; ++p) pair_step(p, NB);
+        for (int p = 0; p < NB; ++p) {
+          const s8 t = tl[p * kPairLanes + tl_lane];
+          pair_step(p, g < 3 ? t : zero);
+        }
+      } else {
+        constexpr int D = kPairRing;
+        static_assert(NB % D == 0, "the pair table has a multiple of kPairRing pairs");
+        s8 ring[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) ring[k] = tg[k * kWave + lane];
+#pragma unroll 1
+        for (int p = 0; p < NB; p += D) {
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            const s8 bw = ring[k];
+            if (p + k + D < NB) ring[k] = tg[(p + k + D) * kWave + lane];
+            pair_step(p + k, bw);
+          }
+        }
+      }
       // even and odd outer indices sit in lanes 32 apart: add the halves
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
         for (int i = 0; i < 4; ++i) dacc[mb][i] += __shfl_xor(dacc[mb][i], 32);
     } else if constexpr (NS == 2) {
+      s8 bA = tg[lane], bB = tg[kWave + lane];
 #pragma unroll 1
       for (int o0 = 0; o0 < R; ++o0) {
         float s1[4];
@@ -750,7 +812,7 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
     tile[R * kDP + lane] = x;
     wave_sync();
     dump_tile<R>(a, tile, tb, cnt, lane);
-    gram.add_tile(tile, s_row[w], tb, cnt);
+    gram.add_tile(tile, s_rowv, tb, cnt);
     wave_sync();
   }
   gram.finish();
@@ -787,6 +849,64 @@ template <int R> void launch_valu(const GramArgs &a, hipStream_t st) {
   hipLaunchKernelGGL((k_gram_valu<R>), dim3(grid), dim3(64 * wpb), per_wave * wpb, st, a, wpb);
 }
 
+// Whether the LDS-table variant of k_gram_bf16_o4 has kO4MaxWaves waves a
+// workgroup (the pair table's copy beside their outer rows and tiles).
+template <int R> bool o4_table_fits(int slots) {
+  if constexpr (!bf16r_pair_table(R)) {
+    return false;
+  } else {
+    if (slots == 2)
+      return o4_table_bytes<R, 2>() +
+                 kO4MaxWaves * o4_lds_floats_per_wave<R, 2>() * sizeof(float) <= lds_limit();
+    return o4_table_bytes<R, 3>() +
+               kO4MaxWaves * o4_lds_floats_per_wave<R, 3>() * sizeof(float) <= lds_limit();
+  }
+}
+
+// Whether k_gram_bf16_o4 runs its LDS-table variant for `a`: the default at
+// order 4 (9.3 against 9.9 ms on the order-4 workload), the L2 one at order 5
+// (21.0 against 23.2 ms on the order-5 one); a.alt_source swaps the two, and
+// a table that does not fit leaves the L2 one.
+template <int R> bool o4_uses_table(const GramArgs &a) {
+  if constexpr (!bf16r_pair_table(R)) {
+    return false;
+  } else {
+    return ((a.slots == 2) != (a.alt_source != 0)) && o4_table_fits<R>(a.slots);
+  }
+}
+
+// Launches k_gram_bf16_o4 over the chunks of `a` (if any), the variant
+// o4_uses_table picks; false if no variant is built for its order.
+template <int R, int KIND> bool launch_o4(const GramArgs &a, int O, hipStream_t st, bool any) {
+  (void)O;
+  auto go = [&](auto ns_tag, auto tl_tag, int wpb) {
+    constexpr int NS = decltype(ns_tag)::value;
+    constexpr bool TL = decltype(tl_tag)::value;
+    const size_t lds = (TL ? o4_table_bytes<R, NS>() : 0) +
+                       (size_t)wpb * o4_lds_floats_per_wave<R, NS>() * sizeof(float);
+    if (lds > lds_limit()) return false;
+    if (any) {
+      const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
+      hipLaunchKernelGGL((k_gram_bf16_o4<R, NS, TL>), dim3(grid), dim3(64 * wpb), lds, st, a,
+                         wpb);
+    }
+    return true;
+  };
+  using NS2 = std::integral_constant<int, 2>;
+  using NS3 = std::integral_constant<int, 3>;
+  using Y = std::true_type;
+  using Nn = std::false_type;
+  if constexpr (bf16r_pair_table(R)) {
+    if (o4_uses_table<R>(a)) {
+      if (a.slots == 2) return go(NS2{}, Y{}, kO4MaxWaves);
+      return go(NS3{}, Y{}, kO4MaxWaves);
+    }
+    if (a.slots == 3) return go(NS3{}, Nn{}, 4);
+  }
+  if (a.slots == 2) return go(NS2{}, Nn{}, 4);
+  return false;
+}
+
 // Picks the delta + Gram kernel for rank R and the requested delta kind, and
 // launches it over the chunks of `a` (if any); returns the gt_delta_path
 // that ran.  A BF16 request falls back to the FP32 delta only where no MFMA
@@ -807,28 +927,17 @@ template <int R, int KIND> int launch_impl(const GramArgs &a, int O, hipStream_t
     const size_t wg4 = per_wave * 4 + table + kStaticLdsBf16;
     const bool table_fits =
         wg4 <= lds_limit() && (int)(lds_limit() / wg4) * 4 >= kMinLdsWavesPerCu;
-    if (!table_fits || a.As_L == nullptr || (KIND == kBf16R && a.prefer_l2)) {
-      // the core operand from L2 instead
+    if (!table_fits || a.As_L == nullptr || (KIND == kBf16R && a.alt_source)) {
+      // the core table does not go in LDS beside the waves (or the autotuner
+      // chose the other source)
       if constexpr (KIND == kBf16R && bf16r_single_mfma(R) && R >= 2) {
-        // orders 4 and 5: the core operand from L2, the outer rows in LDS
-        const int wpb = 4;
-        const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
-        const size_t rows_lds = (size_t)a.slots * R * kWave * sizeof(float);
-        if (a.As_L != nullptr && wpb * rows_lds + kStaticLdsBf16 <= lds_limit()) {
-          if (a.slots == 2 && O == R * R) {
-            if (any)
-              hipLaunchKernelGGL((k_gram_bf16_o4<R, 2>), dim3(grid), dim3(64 * wpb),
-                                 rows_lds * wpb, st, a, wpb);
-            return GT_DELTA_BF16R_L2;
-          }
-          if constexpr (bf16r_pair_table(R)) {
-            if (a.slots == 3 && O == R * R * R) {
-              if (any)
-                hipLaunchKernelGGL((k_gram_bf16_o4<R, 3>), dim3(grid), dim3(64 * wpb),
-                                   rows_lds * wpb, st, a, wpb);
-              return GT_DELTA_BF16R_L2;
-            }
-          }
+        // orders 4 and 5: the outer rows in LDS, the core operand from a copy
+        // of the pair table in LDS or from L2, as o4_uses_table picks
+        const bool order_ok = (a.slots == 2 && O == R * R) ||
+                              (bf16r_pair_table(R) && a.slots == 3 && O == R * R * R);
+        if (a.As_L != nullptr && order_ok) {
+          if (launch_o4<R, KIND>(a, O, st, any))
+            return o4_uses_table<R>(a) ? GT_DELTA_BF16R_O4LDS : GT_DELTA_BF16R_L2;
         }
       }
       launch_valu<R>(a, st);

@@ -182,8 +182,9 @@ struct GramArgs {
   const float *A_O[kMaxOrder - 2];
   const uint32_t *As_L;                  // split BF16 copy of A_L (16 words per row) or null
   float *delta_out;                      // test hook: null, or nnz x R deltas in stream order
-  int prefer_l2;                         // 1: refined BF16 takes the core operand from L2 even
-                                         // where its table fits in LDS (the autotuner's choice)
+  int alt_source;                        // 1: refined BF16 takes its core operand from the other
+                                         // source than the default (LDS table or L2), where
+                                         // both are built (the autotuner's choice)
 };
 void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s);
 void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);
@@ -224,9 +225,9 @@ __host__ __device__ constexpr size_t gb_k32_words(int O) { return (size_t)O * kW
 // refined words of outer indices 2p and 2p + 1 in output rows 0..7 and 8..15,
 // so one MFMA per 16 nonzeros covers two outer indices.
 __host__ __device__ constexpr bool bf16r_pair_table(int R) { return R <= 8; }
-// pairs padded to an even count (zero words), so the kernel's loop over them
-// unrolls by two
-__host__ __device__ constexpr int gb_pairs(int O) { return ((O + 1) / 2 + 1) / 2 * 2; }
+// pairs padded to a multiple of four (zero words), so the kernel's loop over
+// them unrolls by four with a four-deep ring of core operands
+__host__ __device__ constexpr int gb_pairs(int O) { return ((O + 1) / 2 + 3) / 4 * 4; }
 __host__ __device__ constexpr size_t gb_k32_pair_words(int O) {
   return (size_t)gb_pairs(O) * kWave * 8;
 }

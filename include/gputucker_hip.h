@@ -56,8 +56,10 @@ typedef enum gt_delta_path {
   GT_DELTA_BF16 = 3,      /* BF16 MFMA delta, core table in LDS */
   GT_DELTA_BF16R = 4,     /* refined BF16 MFMA delta, core table in LDS */
   GT_DELTA_BF16R_L2 = 5,  /* refined BF16 MFMA delta, core operand from L2 (orders 4-5) */
-  GT_DELTA_BF16R_WIDE = 6 /* refined BF16 MFMA delta of order 3 at 16 <= R <= 32, core operand
+  GT_DELTA_BF16R_WIDE = 6, /* refined BF16 MFMA delta of order 3 at 16 <= R <= 32, core operand
                              from L2, lane-per-entry FP64 Gram */
+  GT_DELTA_BF16R_O4LDS = 7 /* refined BF16 MFMA delta at orders 4-5 (R <= 8), the core pair table
+                              copied to LDS per workgroup of 12 waves */
 } gt_delta_path;
 
 typedef struct gt_opts {
@@ -127,7 +129,9 @@ typedef struct gt_timings {
   uint64_t gram_kernel_launches;
   double ms_iteration_graph;      /* opts.graph: the last replayed iteration, without the fit copy */
   int tuned_chunk_tiles[GT_MAX_ORDER]; /* gt_autotune: 64-nonzero tiles per chunk, per mode (0: none) */
-  int tuned_variant[GT_MAX_ORDER];     /* gt_autotune: 1 = core operand from L2 chosen */
+  int tuned_variant[GT_MAX_ORDER];     /* gt_autotune: 1 = the other core-operand source
+                                          (L2 or LDS table) than the default chosen; the
+                                          delta_path field names the kernel that runs */
   double tuned_ms[GT_MAX_ORDER];       /* gt_autotune: the chosen delta + Gram pass time */
   int delta_path[GT_MAX_ORDER];   /* gt_delta_path of the last update of each mode: a BF16
                                      request that no MFMA kernel covers runs GT_DELTA_FP32 */
