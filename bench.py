@@ -71,11 +71,13 @@ HBM_PEAK_GBS = 8000.0
 # the delta + Gram kernel behind each gt_delta_path, as rocprofv3 names it
 _PATH_KERNEL = {"fp32": "k_gram_valu<{R}>", "fp32-wide": "k_gram_fp32<{R}>",
                 "bf16": "k_gram_bf16<{R}, 1>", "bf16r": "k_gram_bf16<{R}, 2>",
-                "bf16r-l2": "k_gram_bf16_o4<{R}, {NS}, false>", "bf16r-wide": "k_gram_bf16w<{R}>",
-                "bf16r-o4lds": "k_gram_bf16_o4<{R}, {NS}, true>"}
+                "bf16r-l2": "k_gram_bf16_o5<{R}>", "bf16r-wide": "k_gram_bf16w<{R}>",
+                "bf16r-ob": "k_gram_bf16_ob<{R}, {NS}, true>",
+                "bf16r-ob-l2": "k_gram_bf16_ob<{R}, {NS}, false>"}
 # the arithmetic each path's delta runs in
 _PATH_DTYPE = {"fp32": "fp32", "fp32-wide": "fp32", "bf16": "bf16", "bf16r": "bf16r",
-               "bf16r-l2": "bf16r", "bf16r-wide": "bf16r", "bf16r-o4lds": "bf16r"}
+               "bf16r-l2": "bf16r", "bf16r-wide": "bf16r", "bf16r-ob": "bf16r",
+               "bf16r-ob-l2": "bf16r"}
 
 
 def committed_counters(workload, kernel):

@@ -22,7 +22,7 @@ MAX_ORDER = 8
 PRECISIONS = {"fp32": 0, "bf16": 1, "bf16r": 2}
 # gt_delta_path: the delta + Gram kernel a mode update ran
 DELTA_PATHS = {0: "none", 1: "fp32", 2: "fp32-wide", 3: "bf16", 4: "bf16r", 5: "bf16r-l2",
-               6: "bf16r-wide", 7: "bf16r-o4lds"}
+               6: "bf16r-wide", 7: "bf16r-ob", 8: "bf16r-ob-l2"}
 STATUS = {0: "GT_OK", 1: "GT_ERR_ARG", 2: "GT_ERR_HIP", 3: "GT_ERR_STATE",
           4: "GT_ERR_UNSUPPORTED", 5: "GT_ERR_MEMORY", 6: "GT_ERR_PARTITION"}
 
@@ -44,7 +44,7 @@ class GtOpts(C.Structure):
         ("max_iters", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
         ("partition", C.c_uint32 * MAX_ORDER), ("device_budget", C.c_uint64),
         ("chunk_tiles", C.c_int), ("profile", C.c_int), ("ref_fit", C.c_int),
-        ("autotune", C.c_int), ("graph", C.c_int),
+        ("autotune", C.c_int), ("graph", C.c_int), ("core_source", C.c_int),
     ]
 
 
@@ -178,7 +178,7 @@ class Engine:
 
     def __init__(self, order, rank, device=0, lam=None, max_iters=3, tol=None, precision="fp32",
                  partition=None, device_budget=0, chunk_tiles=0, profile=False, ref_fit=False,
-                 graph=False, autotune=False):
+                 graph=False, autotune=False, core_source=0):
         L = lib()
         o = GtOpts()
         L.gt_default_opts(C.byref(o))
@@ -198,6 +198,7 @@ class Engine:
         o.ref_fit = 1 if ref_fit else 0
         o.graph = 1 if graph else 0
         o.autotune = 1 if autotune else 0
+        o.core_source = int(core_source)
         self.opts = o
         self.order, self.rank = int(order), int(rank)
         self._h = C.c_void_p()

@@ -467,7 +467,8 @@ std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R
   const int O = (int)ipow(R, N > 2 ? N - 2 : 0);
   const size_t n16 = gb_k16_words(O), n32 = gb_k32_words(O);
   const size_t npair = bf16r_pair_table(R) ? gb_k32_pair_words(O) : 0;
-  std::vector<uint16_t> tab(n16 + 2 * n32 + npair, 0);
+  const size_t nob = bf16r_ob_table(R, N, O) ? gb_ob_words(R, O) : 0;
+  std::vector<uint16_t> tab(n16 + 2 * n32 + npair + nob, 0);
   auto g_at = [&](int o, int jL, int r) -> float {
     return (o < O && jL < R && r < R) ? gt[((size_t)o * R + jL) * R + r] : 0.f;
   };
@@ -510,6 +511,14 @@ std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R
       const int g = l / 16, q = l % 16;
       k32_words(tab.data() + n16 + 2 * n32 + (p * kWave + l) * 8, (int)(2 * p) + q / 8, g, q % 8);
     }
+  // o-block table (orders 4-5, O <= kObMaxO): entry (ob, r, lane) holds
+  // output r of outer index 16 ob + l % 16
+  const size_t ob_at = gb_ob_offset(R, O);
+  for (int ob = 0; nob && ob < gb_oblocks(O); ++ob)
+    for (int r = 0; r < R; ++r)
+      for (int l = 0; l < kWave; ++l)
+        k32_words(tab.data() + ob_at + (((size_t)ob * R + r) * kWave + l) * 8, 16 * ob + l % 16,
+                  l / 16, r);
   return tab;
 }
 
@@ -1731,6 +1740,10 @@ int gt_create(gt_ctx **out, const gt_opts *opts) {
     g_create_error = "unknown precision";
     return GT_ERR_ARG;
   }
+  if (opts->core_source != 0 && opts->core_source != 1) {
+    g_create_error = "core_source must be 0 or 1";
+    return GT_ERR_ARG;
+  }
   if (opts->precision == GT_PREC_BF16 && !gram_mfma_supported(opts->rank)) {
     g_create_error = "the plain BF16 MFMA delta is built for rank <= 15 (bf16r covers the wide "
                      "ranks of order 3)";
@@ -1986,7 +1999,7 @@ int gt_set_init(gt_ctx *ctx, const float *const *A, const float *core) {
     ctx->initialized = true;
     for (int n = 0; n < GT_MAX_ORDER; ++n) {
       ctx->mchunk[n] = 0;
-      ctx->mvariant[n] = 0;
+      ctx->mvariant[n] = ctx->o.core_source;
     }
     if (ctx->o.autotune) autotune(ctx);
   });

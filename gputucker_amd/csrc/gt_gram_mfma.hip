@@ -36,6 +36,8 @@
 // the zero eigenvalues of sum dd^T by about lambda.  The accumulators are
 // flushed on row change (complete rows to the Gram table, chunk-straddling
 // runs to the partial tables for the fix-up).
+#include <utility>
+
 #include "gt_bf16.hpp"
 #include "gt_contract.hpp"
 
@@ -586,64 +588,45 @@ k_gram_bf16(GramArgs a, int wpb, int O) {
   gram.finish();
 }
 
-// ------------------------------------------------------------ BF16, orders 4-5
-// The refined BF16 delta of an order-4 or order-5 tensor (NS = 2 or 3 outer
-// slots) whose per-wave s table does not leave the order-3 kernel enough
-// waves (order 4, R = 8: 64 outer indices; order 5 needs R <= 8).  The core
-// operand of each MFMA comes either from the table in global memory (it stays
-// in L2), kPairRing steps ahead, or (TL, pair table only) from a copy of the
-// pair table in LDS that the workgroup's waves share; s_m[o] =
-// prod_t A_t[i_t][o_t] is formed at its use from the outer rows staged in
-// LDS.  The next tile's gathers (split A_L words and the outer rows) are
-// issued after this tile's stage loop, so the core loads never wait behind
-// them.
+// ------------------------------------------------------------ BF16, order 5, pairs
+// The refined BF16 delta of an order-5 tensor with more outer indices than
+// the o-block kernel below takes (R = 6..8: 216 to 512), one MFMA per 16
+// nonzeros and two outer indices (the pair table).  The core operand of each
+// MFMA comes from the table in global memory (it stays in L2), kPairRing
+// steps ahead; s_m[o] = prod_t A_t[i_t][o_t] is formed at its use from the
+// outer rows staged in LDS.  The next tile's gathers (split A_L words and the
+// outer rows) are issued after this tile's stage loop, so the core loads
+// never wait behind them.  (The pair table, 83 KiB at R = 6, does not fit in
+// LDS beside enough waves.)
 // core operands in flight in the pair loop (a ring of registers, one per step)
 constexpr int kPairRing = 2;
-// lanes of the pair table that hold words: lane group 3 is zero for R <= 8
-// (bf16r_plan), so the LDS copy keeps lanes 0..47 of each pair
-constexpr int kPairLanes = 48;
-// largest workgroup of the LDS-table variant: 12 waves, three per SIMD (the
-// VGPR budget of amdgpu_waves_per_eu(3))
-constexpr int kO4MaxWaves = 12;
+// the workgroup of a kernel whose waves share one table copy in LDS: 12
+// waves, three per SIMD (the VGPR budget of amdgpu_waves_per_eu(3))
+constexpr int kTableWaves = 12;
 
-// LDS floats per wave of k_gram_bf16_o4: the outer rows, the delta tile, the
+// LDS floats per wave of k_gram_bf16_o5: the outer rows, the delta tile, the
 // tile's row ids
-template <int R, int NS> constexpr size_t o4_lds_floats_per_wave() {
-  return (size_t)NS * R * kWave + 16 * kDP + kWave;
-}
-// LDS bytes of the pair table's copy (TL variant)
-template <int R, int NS> constexpr size_t o4_table_bytes() {
-  return (size_t)gb_pairs(cpow(R, NS)) * kPairLanes * 16;
+template <int R> constexpr size_t o5_lds_floats_per_wave() {
+  return (size_t)3 * R * kWave + 16 * kDP + kWave;
 }
 
-template <int R, int NS, bool TL>
-__global__ void __launch_bounds__(64 * kO4MaxWaves) __attribute__((amdgpu_waves_per_eu(3)))
-k_gram_bf16_o4(GramArgs a, int wpb) {
-  static_assert(bf16r_single_mfma(R) && R >= 2, "one refined MFMA per block and o");
-  static_assert(NS == 2 || (NS == 3 && bf16r_pair_table(R)), "orders 4 and 5 (R <= 8)");
-  static_assert(!TL || bf16r_pair_table(R), "the LDS table variant takes the pair table");
+template <int R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_gram_bf16_o5(GramArgs a, int wpb) {
+  static_assert(bf16r_pair_table(R) && cpow(R, 3) > kObMaxO,
+                "one refined MFMA per two outer indices, beyond the o-block kernel");
+  constexpr int NS = 3;
   constexpr int RS = row_stride(R);
   constexpr int T = ChunkGram<R>::T;
   constexpr int O = cpow(R, NS);
+  constexpr int NB = gb_pairs(O); // MFMAs per 16 nonzeros
   extern __shared__ __align__(16) float dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
-  // R <= 8: the pair table, one MFMA per two outer indices; else the K32 table
-  constexpr bool kPair = bf16r_pair_table(R);
-  constexpr int NB = kPair ? gb_pairs(O) : O; // MFMAs per 16 nonzeros
-  const s8 *tg = reinterpret_cast<const s8 *>(
-      static_cast<const uint16_t *>(a.Gb) + gb_k16_words(O) + (kPair ? 2 * gb_k32_words(O) : 0));
-  constexpr size_t kTabFloats = TL ? o4_table_bytes<R, NS>() / sizeof(float) : 0;
-  const s8 *tl = reinterpret_cast<const s8 *>(dyn);
-  if constexpr (TL) { // the pair table's nonzero lanes to LDS, once per workgroup
-    uint4 *dst = reinterpret_cast<uint4 *>(dyn);
-    const uint4 *src = reinterpret_cast<const uint4 *>(tg);
-    for (int i = threadIdx.x; i < NB * kPairLanes; i += blockDim.x)
-      dst[i] = src[(i / kPairLanes) * kWave + i % kPairLanes];
-    __syncthreads();
-  }
+  const s8 *tg = reinterpret_cast<const s8 *>(static_cast<const uint16_t *>(a.Gb) +
+                                              gb_k16_words(O) + 2 * gb_k32_words(O));
   const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
   if (c >= a.chunk_end) return;
-  float *lrows = dyn + kTabFloats + (size_t)w * o4_lds_floats_per_wave<R, NS>(); // [slot][j][nonzero]
+  float *lrows = dyn + (size_t)w * o5_lds_floats_per_wave<R>(); // [slot][j][nonzero]
   float *tile = lrows + NS * R * kWave;
   uint32_t *s_rowv = reinterpret_cast<uint32_t *>(tile + 16 * kDP);
   const uint64_t cb = c * a.chunk;
@@ -700,116 +683,286 @@ k_gram_bf16_o4(GramArgs a, int wpb) {
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
-    if constexpr (kPair) {
-      // MFMA p covers outer indices 2p (output rows 0..7) and 2p + 1 (rows
-      // 8..15): lane (q, g) gets U[16mb + q][o][r] with o = 2p + g / 2 and
-      // r = 4 (g % 2) + i, so it scales by s_m[o] of its own o.  The digits
-      // of both outer indices of a step are wave-uniform (scalar work); a
-      // lane only selects its half's.  The core operands come from L2
-      // kPairRing steps ahead, through a ring of registers.
-      const bool odd = g >= 2;
-      auto pair_step = [&](int p, const s8 &bw) {
-        const b8 b = __builtin_bit_cast(b8, bw);
-        f4 u[4];
-        const f4 z = {0.f, 0.f, 0.f, 0.f};
+    // MFMA p covers outer indices 2p (output rows 0..7) and 2p + 1 (rows
+    // 8..15): lane (q, g) gets U[16mb + q][o][r] with o = 2p + g / 2 and
+    // r = 4 (g % 2) + i, so it scales by s_m[o] of its own o.  The digits
+    // of both outer indices of a step are wave-uniform (scalar work); a
+    // lane only selects its half's.
+    const bool odd = g >= 2;
+    auto pair_step = [&](int p, const s8 &bw) {
+      const b8 b = __builtin_bit_cast(b8, bw);
+      f4 u[4];
+      const f4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb)
-          u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, g32[mb]), z, 0,
-                                                          0, 0);
-        // past O the core words are zero: any in-range outer index will do
-        const int oe = min(2 * p, O - 1), oo = min(2 * p + 1, O - 1);
-        const float *l[NS];
+      for (int mb = 0; mb < 4; ++mb)
+        u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, g32[mb]), z, 0, 0,
+                                                        0);
+      // past O the core words are zero: any in-range outer index will do
+      const int oe = min(2 * p, O - 1), oo = min(2 * p + 1, O - 1);
+      const float *l[NS];
 #pragma unroll
-        for (int t = 0; t < NS; ++t) {
-          // both offsets are computed in scalar registers and hidden from the
-          // optimiser: otherwise it folds the select into the digits of a
-          // per-lane 2p + half, with vector divisions by R at every step
-          int offe = (t * R + oe / cpow(R, NS - 1 - t) % R) * kWave;
-          int offo = (t * R + oo / cpow(R, NS - 1 - t) % R) * kWave;
-          asm volatile("" : "+s"(offe), "+s"(offo));
-          l[t] = lrows + q + (odd ? offo : offe);
-        }
+      for (int t = 0; t < NS; ++t) {
+        // both offsets are computed in scalar registers and hidden from the
+        // optimiser: otherwise it folds the select into the digits of a
+        // per-lane 2p + half, with vector divisions by R at every step
+        int offe = (t * R + oe / cpow(R, NS - 1 - t) % R) * kWave;
+        int offo = (t * R + oo / cpow(R, NS - 1 - t) % R) * kWave;
+        asm volatile("" : "+s"(offe), "+s"(offo));
+        l[t] = lrows + q + (odd ? offo : offe);
+      }
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-          float sv = l[0][mb * 16];
+      for (int mb = 0; mb < 4; ++mb) {
+        float sv = l[0][mb * 16];
 #pragma unroll
-          for (int t = 1; t < NS; ++t) sv *= l[t][mb * 16];
+        for (int t = 1; t < NS; ++t) sv *= l[t][mb * 16];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
-        }
-      };
-      if constexpr (TL) {
-        // lane group 3 takes zeros (its words are zero in the table too)
-        const s8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
-        const int tl_lane = g < 3 ? lane : 0;
-#pragma unroll 2
-        for (int p = 0; p < NB; ++p) {
-          const s8 t = tl[p * kPairLanes + tl_lane];
-          pair_step(p, g < 3 ? t : zero);
-        }
-      } else {
-        constexpr int D = kPairRing;
-        static_assert(NB % D == 0, "the pair table has a multiple of kPairRing pairs");
-        s8 ring[D];
+        for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
+      }
+    };
+    {
+      constexpr int D = kPairRing;
+      static_assert(NB % D == 0, "the pair table has a multiple of kPairRing pairs");
+      s8 ring[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) ring[k] = tg[k * kWave + lane];
+      for (int k = 0; k < D; ++k) ring[k] = tg[k * kWave + lane];
 #pragma unroll 1
-        for (int p = 0; p < NB; p += D) {
+      for (int p = 0; p < NB; p += D) {
 #pragma unroll
-          for (int k = 0; k < D; ++k) {
-            const s8 bw = ring[k];
-            if (p + k + D < NB) ring[k] = tg[(p + k + D) * kWave + lane];
-            pair_step(p + k, bw);
-          }
+        for (int k = 0; k < D; ++k) {
+          const s8 bw = ring[k];
+          if (p + k + D < NB) ring[k] = tg[(p + k + D) * kWave + lane];
+          pair_step(p + k, bw);
         }
       }
-      // even and odd outer indices sit in lanes 32 apart: add the halves
+    }
+    // even and odd outer indices sit in lanes 32 apart: add the halves
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dacc[mb][i] += __shfl_xor(dacc[mb][i], 32);
+    // the next tile's gathers and the stream loads of the tile after it
+    gather();
+    load_idx(tb + 2 * kWave);
+    // lane (q, g) holds delta[16mb + q][4g + i] for g < 2 (columns 8 and up
+    // are zero): columns to LDS, then x
+    if (g < T) {
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) dacc[mb][i] += __shfl_xor(dacc[mb][i], 32);
-    } else if constexpr (NS == 2) {
-      s8 bA = tg[lane], bB = tg[kWave + lane];
-#pragma unroll 1
-      for (int o0 = 0; o0 < R; ++o0) {
-        float s1[4];
+        for (int i = 0; i < 4; ++i) tile[(4 * g + i) * kDP + mb * 16 + q] = g < 2 ? dacc[mb][i] : 0.f;
+    }
+    tile[R * kDP + lane] = x;
+    wave_sync();
+    dump_tile<R>(a, tile, tb, cnt, lane);
+    gram.add_tile(tile, s_rowv, tb, cnt);
+    wave_sync();
+  }
+  gram.finish();
+}
+
+// ------------------------------------------------------------ BF16, o-blocks
+// The refined BF16 delta of an order-4 or order-5 tensor with at most
+// kObMaxO outer indices (R <= 10), one MFMA per 16 nonzeros, 16 outer
+// indices and output column r: A = block (ob, r) of the o-block table (row q
+// holds the refined words of G[16 ob + q][.][r]), B = the split A_L words of
+// 16 nonzeros, so lane (q, g) gets U[16 mb + q][16 ob + 4g + i][r], i < 4.
+// Stage 2 scales those by s_m[o]: each lane forms s for its own nonzero and
+// the block's 16 outer indices (o is wave-uniform and the loop over o is
+// unrolled in full, so the outer rows are register reads), stores them to a
+// per-wave LDS block [nonzero][o] and reads back the four of its MFMA
+// layout with one 16-byte load per block of 16 nonzeros.  Each lane group
+// accumulates over its own outer indices; a butterfly over the four groups
+// adds them at the end of the tile.  Beside k_gram_bf16_o5 (one MFMA per two
+// outer indices, output rows (o, r) padded to 8 columns) this runs R/8 of the
+// MFMAs and scaling FMAs (the same at R = 8) and about half the s work.  The
+// core operand comes from a copy of the table in LDS (TL, 40 KiB at R = 5,
+// order 5) or from L2.
+// pitch of the per-wave s block in floats: 16 outer indices and 4 of padding,
+// so the 16-byte reads of a block of 16 nonzeros spread over the banks
+constexpr int kObSP = 20;
+template <int R, int NS> constexpr size_t ob_lds_floats_per_wave() {
+  // the s block [64][kObSP] and the delta tile [16][kDP] share a buffer;
+  // then the tile's row ids
+  return (size_t)(kWave * kObSP > 16 * kDP ? kWave * kObSP : 16 * kDP) + kWave;
+}
+template <int R, int NS> constexpr size_t ob_table_bytes() {
+  return (size_t)gb_oblocks(cpow(R, NS)) * R * kWave * 16;
+}
+
+// s of a lane's nonzero for outer indices 16 OB .. 16 OB + 15 (zero past O,
+// where the table's rows are zero too), from its outer rows in registers
+template <int R, int NS, int OB>
+__device__ inline __attribute__((always_inline)) void ob_svalues(
+    const float (&ro)[NS][row_stride(R)], f4 (&sv)[4]) {
+  constexpr int O = cpow(R, NS);
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) s1[mb] = lrows[o0 * kWave + mb * 16 + q];
-#pragma unroll 2
-        for (int o1 = 0; o1 < R; ++o1) {
-          const int o = o0 * R + o1;
-          const b8 b = __builtin_bit_cast(b8, bA);
-          bA = bB;
-          if (o + 2 < O) bB = tg[(o + 2) * kWave + lane];
-          // U transposed: lane (q, g) gets U[16mb + q][o][4g + i]
-          f4 u[4];
+  for (int k4 = 0; k4 < 4; ++k4)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int o = 16 * OB + 4 * k4 + i;
+      float sp = 0.f;
+      if (o < O) {
+        sp = ro[0][o / cpow(R, NS - 1) % R];
+#pragma unroll
+        for (int t = 1; t < NS; ++t) sp *= ro[t][o / cpow(R, NS - 1 - t) % R];
+      }
+      sv[k4][i] = sp;
+    }
+}
+// the same for a wave-uniform block index: one branch per block, each with
+// its digits fixed at compile time (an unrolled loop over the blocks would
+// let the scheduler hoist every block's work and spill)
+template <int R, int NS, int... OB>
+__device__ inline __attribute__((always_inline)) void ob_svalues_at(
+    int ob, const float (&ro)[NS][row_stride(R)], f4 (&sv)[4],
+    std::integer_sequence<int, OB...>) {
+  ((ob == OB ? ob_svalues<R, NS, OB>(ro, sv) : void()), ...);
+}
+
+template <int R, int NS, bool TL>
+__global__ void __launch_bounds__(64 * kTableWaves) __attribute__((amdgpu_waves_per_eu(3)))
+k_gram_bf16_ob(GramArgs a, int wpb) {
+  constexpr int O = cpow(R, NS);
+  constexpr int NOB = gb_oblocks(O);
+  static_assert(bf16r_single_mfma(R) && (NS == 2 || NS == 3) && O <= kObMaxO,
+                "orders 4-5, one refined MFMA per block, at most kObMaxO outer indices");
+  constexpr int RS = row_stride(R);
+  constexpr int T = ChunkGram<R>::T;
+  constexpr size_t kBuf = ob_lds_floats_per_wave<R, NS>() - kWave;
+  extern __shared__ __align__(16) float dyn[];
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
+  const s8 *tg = reinterpret_cast<const s8 *>(static_cast<const uint16_t *>(a.Gb) +
+                                              gb_ob_offset(R, O));
+  constexpr size_t kTabFloats = TL ? ob_table_bytes<R, NS>() / sizeof(float) : 0;
+  if constexpr (TL) { // the o-block table to LDS, once per workgroup
+    uint4 *dst = reinterpret_cast<uint4 *>(dyn);
+    const uint4 *src = reinterpret_cast<const uint4 *>(tg);
+    for (int i = threadIdx.x; i < NOB * R * kWave; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+  }
+  const s8 *tab = TL ? reinterpret_cast<const s8 *>(dyn) : tg;
+  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
+  if (c >= a.chunk_end) return;
+  float *buf = dyn + kTabFloats + (size_t)w * ob_lds_floats_per_wave<R, NS>();
+  float *tile = buf; // after the stage loop; the s block before it
+  uint32_t *s_rowv = reinterpret_cast<uint32_t *>(buf + kBuf);
+  const uint64_t cb = c * a.chunk;
+  const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
+  const int q = lane & 15, g = lane >> 4;
+  ChunkGram<R> gram(a, c, cb, ce, lane);
+
+  uint32_t n_row, n_iL, n_io[NS];
+  float n_x;
+  auto load_idx = [&](uint64_t tb) {
+    const uint64_t e = tb + lane;
+    const uint64_t ee = e < ce ? e : cb;
+    n_row = a.idx_row[ee];
+    n_iL = a.idx_L[ee];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) n_io[t] = a.idx_O[t][ee];
+    n_x = e < ce ? a.st.val[ee] : 0.f;
+  };
+  // lane (q, g) takes, for block mb, the split words of nonzero 16 mb + q;
+  // each lane gathers its own nonzero's outer rows
+  u4 g32[4];
+  float ro[NS][RS];
+  auto gather = [&]() {
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const uint32_t r = (uint32_t)__shfl((int)n_iL, mb * 16 + q);
+      g32[mb] = *reinterpret_cast<const u4 *>(a.As_L + ((size_t)r * 4 + g) * 4);
+    }
+#pragma unroll
+    for (int t = 0; t < NS; ++t) load_row<R>(a.A_O[t], n_io[t], ro[t]);
+  };
+  load_idx(cb);
+  uint32_t c_row = n_row;
+  float c_x = n_x;
+  gather();
+  load_idx(cb + kWave);
+
+  for (uint64_t tb = cb; tb < ce; tb += kWave) {
+    const bool valid = tb + lane < ce;
+    const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
+    const uint32_t row = c_row;
+    const float x = c_x;
+    c_row = n_row;
+    c_x = n_x;
+    s_rowv[lane] = valid ? row : 0xFFFFFFFFu;
+    float dacc[4][R];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int r = 0; r < R; ++r) dacc[mb][r] = 0.f;
+#pragma unroll 1
+    for (int ob = 0; ob < NOB; ++ob) {
+      // the outer rows as values of this iteration: otherwise every block's
+      // products are hoisted out of the loop, O registers that spill
+#pragma unroll
+      for (int t = 0; t < NS; ++t)
+#pragma unroll
+        for (int j = 0; j < R; ++j) asm volatile("" : "+v"(ro[t][j]));
+      f4 sv[4];
+      ob_svalues_at<R, NS>(ob, ro, sv, std::make_integer_sequence<int, NOB>{});
+      wave_sync(); // the previous block's reads of the s block are done
+#pragma unroll
+      for (int k4 = 0; k4 < 4; ++k4)
+        *reinterpret_cast<f4 *>(buf + lane * kObSP + 4 * k4) = sv[k4];
+      wave_sync();
+      // lane (q, g) scales outer indices 16 ob + 4g + i of nonzero 16 mb + q
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+        sv[mb] = *reinterpret_cast<const f4 *>(buf + (mb * 16 + q) * kObSP + 4 * g);
+      // column r's four MFMAs issue before column r - 1's scaling, so the
+      // scaling never waits on a result just issued
+      const s8 *tb_ob = tab + (size_t)ob * R * kWave + lane;
+      f4 up[4];
+#pragma unroll
+      for (int r = 0; r <= R; ++r) {
+        f4 un[4];
+        if (r < R) {
+          const b8 b = __builtin_bit_cast(b8, tb_ob[r * kWave]);
           const f4 z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int mb = 0; mb < 4; ++mb)
-            u[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, g32[mb]), z,
-                                                            0, 0, 0);
+            un[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, __builtin_bit_cast(b8, g32[mb]), z,
+                                                             0, 0, 0);
+        }
+        if (r > 0) {
 #pragma unroll
-          for (int mb = 0; mb < 4; ++mb) {
-            const float sv = s1[mb] * lrows[(R + o1) * kWave + mb * 16 + q];
+          for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(u[mb][i], sv, dacc[mb][i]);
-          }
+            for (int i = 0; i < 4; ++i)
+              dacc[mb][r - 1] = __builtin_fmaf(up[mb][i], sv[mb][i], dacc[mb][r - 1]);
+        }
+        if (r < R) {
+#pragma unroll
+          for (int mb = 0; mb < 4; ++mb) up[mb] = un[mb];
         }
       }
     }
     // the next tile's gathers and the stream loads of the tile after it
     gather();
     load_idx(tb + 2 * kWave);
-    // lane (q, g) holds delta[16mb + q][4g + i] (with the pair table, for
-    // g < 2; columns 8 and up are zero): columns to LDS, then x
-    if (g < T) {
+    // add the four lane groups' partial sums (disjoint outer indices)
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
+    for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-          tile[(4 * g + i) * kDP + mb * 16 + q] = (!kPair || g < 2) ? dacc[mb][i] : 0.f;
+      for (int r = 0; r < R; ++r) {
+        dacc[mb][r] += __shfl_xor(dacc[mb][r], 16);
+        dacc[mb][r] += __shfl_xor(dacc[mb][r], 32);
+      }
+    wave_sync(); // the s block's reads are done before the tile overwrites it
+    // lane (q, g) writes the deltas of nonzero 16 g + q = lane, then x over
+    // column R and zeros up to the Gram's last column tile
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const float v = g == 0 ? dacc[0][r] : g == 1 ? dacc[1][r] : g == 2 ? dacc[2][r] : dacc[3][r];
+      tile[r * kDP + lane] = v;
     }
     tile[R * kDP + lane] = x;
+#pragma unroll
+    for (int r = R + 1; r < 4 * T; ++r) tile[r * kDP + lane] = 0.f;
     wave_sync();
     dump_tile<R>(a, tile, tb, cnt, lane);
     gram.add_tile(tile, s_rowv, tb, cnt);
@@ -849,62 +1002,49 @@ template <int R> void launch_valu(const GramArgs &a, hipStream_t st) {
   hipLaunchKernelGGL((k_gram_valu<R>), dim3(grid), dim3(64 * wpb), per_wave * wpb, st, a, wpb);
 }
 
-// Whether the LDS-table variant of k_gram_bf16_o4 has kO4MaxWaves waves a
-// workgroup (the pair table's copy beside their outer rows and tiles).
-template <int R> bool o4_table_fits(int slots) {
-  if constexpr (!bf16r_pair_table(R)) {
+// Launches k_gram_bf16_o5 over the chunks of `a` (if any); false where it is
+// not built for R (below R = 6 the o-block kernel takes order 5).
+template <int R> bool launch_o5(const GramArgs &a, hipStream_t st, bool any) {
+  if constexpr (!(bf16r_pair_table(R) && cpow(R, 3) > kObMaxO)) {
     return false;
   } else {
-    if (slots == 2)
-      return o4_table_bytes<R, 2>() +
-                 kO4MaxWaves * o4_lds_floats_per_wave<R, 2>() * sizeof(float) <= lds_limit();
-    return o4_table_bytes<R, 3>() +
-               kO4MaxWaves * o4_lds_floats_per_wave<R, 3>() * sizeof(float) <= lds_limit();
-  }
-}
-
-// Whether k_gram_bf16_o4 runs its LDS-table variant for `a`: the default at
-// order 4 (9.3 against 9.9 ms on the order-4 workload), the L2 one at order 5
-// (21.0 against 23.2 ms on the order-5 one); a.alt_source swaps the two, and
-// a table that does not fit leaves the L2 one.
-template <int R> bool o4_uses_table(const GramArgs &a) {
-  if constexpr (!bf16r_pair_table(R)) {
-    return false;
-  } else {
-    return ((a.slots == 2) != (a.alt_source != 0)) && o4_table_fits<R>(a.slots);
-  }
-}
-
-// Launches k_gram_bf16_o4 over the chunks of `a` (if any), the variant
-// o4_uses_table picks; false if no variant is built for its order.
-template <int R, int KIND> bool launch_o4(const GramArgs &a, int O, hipStream_t st, bool any) {
-  (void)O;
-  auto go = [&](auto ns_tag, auto tl_tag, int wpb) {
-    constexpr int NS = decltype(ns_tag)::value;
-    constexpr bool TL = decltype(tl_tag)::value;
-    const size_t lds = (TL ? o4_table_bytes<R, NS>() : 0) +
-                       (size_t)wpb * o4_lds_floats_per_wave<R, NS>() * sizeof(float);
+    const int wpb = 4;
+    const size_t lds = (size_t)wpb * o5_lds_floats_per_wave<R>() * sizeof(float);
     if (lds > lds_limit()) return false;
     if (any) {
       const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
-      hipLaunchKernelGGL((k_gram_bf16_o4<R, NS, TL>), dim3(grid), dim3(64 * wpb), lds, st, a,
-                         wpb);
+      hipLaunchKernelGGL((k_gram_bf16_o5<R>), dim3(grid), dim3(64 * wpb), lds, st, a, wpb);
     }
     return true;
-  };
-  using NS2 = std::integral_constant<int, 2>;
-  using NS3 = std::integral_constant<int, 3>;
-  using Y = std::true_type;
-  using Nn = std::false_type;
-  if constexpr (bf16r_pair_table(R)) {
-    if (o4_uses_table<R>(a)) {
-      if (a.slots == 2) return go(NS2{}, Y{}, kO4MaxWaves);
-      return go(NS3{}, Y{}, kO4MaxWaves);
-    }
-    if (a.slots == 3) return go(NS3{}, Nn{}, 4);
   }
-  if (a.slots == 2) return go(NS2{}, Nn{}, 4);
-  return false;
+}
+
+// Launches k_gram_bf16_ob over the chunks of `a` (if any) and returns the
+// gt_delta_path it takes: the table in LDS beside twelve waves a workgroup
+// unless a.alt_source or it does not fit, else from L2; -1 where the kernel
+// is not built for R and NS.
+template <int R, int NS> int launch_ob(const GramArgs &a, hipStream_t st, bool any) {
+  constexpr int O = cpow(R, NS);
+  if constexpr (!(bf16r_single_mfma(R) && R >= 2 && O <= kObMaxO)) {
+    return -1;
+  } else {
+    const size_t per_wave = ob_lds_floats_per_wave<R, NS>() * sizeof(float);
+    const bool tl =
+        !a.alt_source && ob_table_bytes<R, NS>() + kTableWaves * per_wave <= lds_limit();
+    const int wpb = tl ? kTableWaves : 4;
+    const size_t lds = (tl ? ob_table_bytes<R, NS>() : 0) + (size_t)wpb * per_wave;
+    if (lds > lds_limit()) return -1;
+    if (any) {
+      const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
+      if (tl)
+        hipLaunchKernelGGL((k_gram_bf16_ob<R, NS, true>), dim3(grid), dim3(64 * wpb), lds, st, a,
+                           wpb);
+      else
+        hipLaunchKernelGGL((k_gram_bf16_ob<R, NS, false>), dim3(grid), dim3(64 * wpb), lds, st, a,
+                           wpb);
+    }
+    return tl ? GT_DELTA_BF16R_OB : GT_DELTA_BF16R_OB_L2;
+  }
 }
 
 // Picks the delta + Gram kernel for rank R and the requested delta kind, and
@@ -919,6 +1059,15 @@ template <int R, int KIND> int launch_impl(const GramArgs &a, int O, hipStream_t
     launch_valu<R>(a, st);
     return GT_DELTA_FP32;
   } else {
+    if constexpr (KIND == kBf16R) {
+      // orders 4-5: o-blocks up to kObMaxO outer indices, then (order 5,
+      // R <= 8) pairs of outer indices
+      if (a.As_L != nullptr && (a.slots == 2 || a.slots == 3) && O == cpow(R, a.slots)) {
+        const int p = a.slots == 2 ? launch_ob<R, 2>(a, st, any) : launch_ob<R, 3>(a, st, any);
+        if (p >= 0) return p;
+        if (a.slots == 3 && launch_o5<R>(a, st, any)) return GT_DELTA_BF16R_L2;
+      }
+    }
     const size_t per_wave = lds_floats_per_wave<KIND>(a.slots, R, O) * sizeof(float);
     const size_t table =
         KIND == kBf16 ? gb_k16_words(O) * 2
@@ -927,19 +1076,7 @@ template <int R, int KIND> int launch_impl(const GramArgs &a, int O, hipStream_t
     const size_t wg4 = per_wave * 4 + table + kStaticLdsBf16;
     const bool table_fits =
         wg4 <= lds_limit() && (int)(lds_limit() / wg4) * 4 >= kMinLdsWavesPerCu;
-    if (!table_fits || a.As_L == nullptr || (KIND == kBf16R && a.alt_source)) {
-      // the core table does not go in LDS beside the waves (or the autotuner
-      // chose the other source)
-      if constexpr (KIND == kBf16R && bf16r_single_mfma(R) && R >= 2) {
-        // orders 4 and 5: the outer rows in LDS, the core operand from a copy
-        // of the pair table in LDS or from L2, as o4_uses_table picks
-        const bool order_ok = (a.slots == 2 && O == R * R) ||
-                              (bf16r_pair_table(R) && a.slots == 3 && O == R * R * R);
-        if (a.As_L != nullptr && order_ok) {
-          if (launch_o4<R, KIND>(a, O, st, any))
-            return o4_uses_table<R>(a) ? GT_DELTA_BF16R_O4LDS : GT_DELTA_BF16R_L2;
-        }
-      }
+    if (!table_fits || a.As_L == nullptr) { // no MFMA variant for this shape
       launch_valu<R>(a, st);
       return GT_DELTA_FP32;
     }

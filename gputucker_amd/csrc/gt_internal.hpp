@@ -231,6 +231,22 @@ __host__ __device__ constexpr int gb_pairs(int O) { return ((O + 1) / 2 + 3) / 4
 __host__ __device__ constexpr size_t gb_k32_pair_words(int O) {
   return (size_t)gb_pairs(O) * kWave * 8;
 }
+// Orders 4 and 5 with at most kObMaxO outer indices (R <= 10 on the single
+// refined MFMA) get a last table, the o-block table ([ob][r][lane][8]): row q
+// of block (ob, r) holds the K32 refined words of G[16 ob + q][.][r] (zero
+// rows for 16 ob + q >= O), so one MFMA per 16 nonzeros covers 16 outer
+// indices of output column r.
+constexpr int kObMaxO = 128;
+__host__ __device__ constexpr int gb_oblocks(int O) { return (O + 15) / 16; }
+__host__ __device__ constexpr bool bf16r_ob_table(int R, int N, int O) {
+  return bf16r_single_mfma(R) && (N == 4 || N == 5) && O <= kObMaxO;
+}
+__host__ __device__ constexpr size_t gb_ob_offset(int R, int O) {
+  return gb_k16_words(O) + 2 * gb_k32_words(O) + (bf16r_pair_table(R) ? gb_k32_pair_words(O) : 0);
+}
+__host__ __device__ constexpr size_t gb_ob_words(int R, int O) {
+  return (size_t)gb_oblocks(O) * R * kWave * 8;
+}
 // Returns the gt_delta_path it took.
 int launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
 // Split BF16 copy of a factor for the MFMA delta (R <= 15): 16 uint32 words

@@ -55,11 +55,14 @@ typedef enum gt_delta_path {
   GT_DELTA_FP32_WIDE = 2, /* FP32 VALU delta, lane-per-entry FP64 Gram (R > 15) */
   GT_DELTA_BF16 = 3,      /* BF16 MFMA delta, core table in LDS */
   GT_DELTA_BF16R = 4,     /* refined BF16 MFMA delta, core table in LDS */
-  GT_DELTA_BF16R_L2 = 5,  /* refined BF16 MFMA delta, core operand from L2 (orders 4-5) */
+  GT_DELTA_BF16R_L2 = 5,  /* refined BF16 MFMA delta of order 5 at R = 6..8, one MFMA per two
+                             outer indices, core operand from L2 */
   GT_DELTA_BF16R_WIDE = 6, /* refined BF16 MFMA delta of order 3 at 16 <= R <= 32, core operand
                              from L2, lane-per-entry FP64 Gram */
-  GT_DELTA_BF16R_O4LDS = 7 /* refined BF16 MFMA delta at orders 4-5 (R <= 8), the core pair table
-                              copied to LDS per workgroup of 12 waves */
+  GT_DELTA_BF16R_OB = 7,   /* refined BF16 MFMA delta at orders 4-5 with R^(N-2) <= 128, one MFMA
+                              per 16 outer indices, the o-block table copied to LDS per
+                              workgroup of 12 waves */
+  GT_DELTA_BF16R_OB_L2 = 8 /* the same, o-block table from L2 */
 } gt_delta_path;
 
 typedef struct gt_opts {
@@ -91,6 +94,10 @@ typedef struct gt_opts {
                                        into a HIP graph on first use and replay it; dropped
                                        by gt_set_coo, gt_partition and gt_set_init.  No per-mode
                                        or per-kernel timings inside it (ms_iteration_graph) */
+  int core_source;                  /* 0: each refined BF16 delta kernel takes its core operand
+                                       from its default source; 1: from the other one (a table
+                                       in LDS or L2) where both are built.  gt_autotune
+                                       chooses per mode instead (tuned_variant) */
 } gt_opts;
 
 typedef struct gt_partition_info {

@@ -36,8 +36,9 @@ def test_kernel_per_delta_path():
     assert bench.kernel_name("bf16", 10) == "k_gram_bf16<10, 1>"
     assert bench.kernel_name("fp32", 10) == "k_gram_valu<10>"
     assert bench.kernel_name("fp32-wide", 32) == "k_gram_fp32<32>"
-    assert bench.kernel_name("bf16r-l2", 8, 4) == "k_gram_bf16_o4<8, 2, false>"
-    assert bench.kernel_name("bf16r-o4lds", 5, 5) == "k_gram_bf16_o4<5, 3, true>"
+    assert bench.kernel_name("bf16r-l2", 6, 5) == "k_gram_bf16_o5<6>"
+    assert bench.kernel_name("bf16r-ob", 5, 5) == "k_gram_bf16_ob<5, 3, true>"
+    assert bench.kernel_name("bf16r-ob-l2", 8, 4) == "k_gram_bf16_ob<8, 2, false>"
 
 
 def test_kernel_rank():

@@ -117,10 +117,10 @@ def test_hot_path_delta_parity(prec, tol, N, R, part):
     FP64 oracle's delta (delta.cuh:62-90) of the same nonzero, relative to
     the largest entry of that nonzero's delta.  Covers the FP32 VALU delta
     (two tiles per pass, and the R > 15 kernel), the refined BF16 MFMA delta
-    (order 3 from the LDS table, with the second MFMA at R = 13, and orders 4
-    and 5 with the core operand from L2, and the wide ranks 16..32 of order 3
-    with the core operand from L2; R = 17 runs zero-padded at 20) and plain
-    BF16."""
+    (order 3 from the LDS table, with the second MFMA at R = 13; orders 4
+    and 5 on the o-block kernel, its table in LDS; the wide ranks 16..32 of
+    order 3 with the core operand from L2, R = 17 zero-padded at 20) and
+    plain BF16."""
     if R > 15 and prec == "bf16":
         pytest.skip("plain BF16 covers R <= 15")
     dims = [90, 70, 50, 30, 20][:N]
@@ -275,12 +275,12 @@ def test_error_paths():
                                       (3, 20, [2, 1, 2]), (3, 32, [1, 1, 2]), (3, 24, [1, 2, 1])])
 def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
     """BF16 MFMA delta (plain and with the split-BF16 refinement; above
-    R = 10 the refinement takes a second MFMA per tile; at order 4 with
-    R = 7 and 9, and at order 5 with R = 4, the refined core table exceeds
-    LDS and the core operand comes from L2, two outer indices per MFMA up to
-    R = 8) and the
-    FP32 VALU delta (two tiles per pass) feeding the MFMA Gram, against the
-    FP64 oracle's B and C."""
+    R = 10 the refinement takes a second MFMA per tile; orders 4 and 5 run
+    the refined delta on the o-block kernel, 16 outer indices per MFMA, at
+    R = 5, 7 and 9 of order 4 and R = 4, 5 of order 5 with partial blocks
+    or none; plain BF16 there takes the order-3 kernel's loop over outer
+    indices) and the FP32 VALU delta (two tiles per pass) feeding the MFMA
+    Gram, against the FP64 oracle's B and C."""
     if R > 15 and prec == "bf16":
         pytest.skip("plain BF16 covers R <= 15")
     dims = [90, 70, 50, 30, 20][:N]
@@ -575,7 +575,7 @@ def test_graph_iteration_equals_eager(prec, N, R, part):
                                            ("fp32", 3, 20, [1, 1, 1])])
 def test_autotune_keeps_parity(prec, N, R, part):
     """gt_autotune times each mode's delta + Gram pass over chunk sizes (and,
-    for refined BF16 at order 4, the core operand from LDS or from L2) and
+    for refined BF16 at order 4, the o-block table in LDS or from L2) and
     keeps the fastest.  The factors are untouched by the tuning, and the ALS
     run that follows agrees with the untuned one to FP64 rounding of the Gram
     sums and with the FP64 oracle to the precision's bar."""
@@ -597,6 +597,44 @@ def test_autotune_keeps_parity(prec, N, R, part):
     assert np.allclose(fits, fits_ref, rtol=1e-7, atol=0), (fits, fits_ref)
     f64, _ = oracle.als(t, A, G, 1e-3, 3, 0.0, "f64")
     assert (np.abs(fits - f64) / np.abs(f64)).max() < (1e-5 if prec == "fp32" else 1e-4)
+
+
+@pytest.mark.parametrize("N,R,part,paths", [
+    (4, 5, [1, 2, 1, 1], ("bf16r-ob", "bf16r-ob-l2")),
+    (4, 8, [1, 1, 2, 1], ("bf16r-ob", "bf16r-ob-l2")),
+    (4, 10, [1, 1, 1, 2], ("bf16r-ob", "bf16r-ob-l2")),
+    (5, 5, [1, 1, 1, 2, 1], ("bf16r-ob", "bf16r-ob-l2")),
+    (5, 2, [1, 1, 1, 1, 2], ("bf16r-ob", "bf16r-ob-l2")),
+    (5, 6, [1, 1, 2, 1, 1], ("bf16r-l2", "bf16r-l2"))])
+def test_core_source_variants(N, R, part, paths):
+    """Both sources of the refined BF16 core operand (core_source 0 and 1):
+    the o-block kernel (orders 4-5 with R^(N-2) <= 128, the o-block table in
+    LDS or from L2, with partial last blocks at R = 5 and 10) and, above
+    that, the pair kernel (its pair table, 83 KiB at order 5 and R = 6, does
+    not go in LDS beside twelve waves: L2 either way).  Each one's fused
+    deltas against the FP64 oracle's (delta.cuh:62-90), and its B and C
+    against the oracle's (update.cuh:35-101), at the refined BF16 bars."""
+    dims = [90, 70, 50, 30, 20][:N]
+    for src, want in enumerate(paths):
+        print(f"core source {src}: N={N} R={R}", flush=True)
+        e, t, A, G = _setup(N * 29 + R, dims, 9000, R, part, skew=1.2, precision="bf16r",
+                            chunk_tiles=1, core_source=src)
+        lam = gt.default_lambda()
+        scale_n = max(1, R ** (N - 1) / 100)
+        for m in range(N):
+            d, idx_s = e.get_stream_delta(m)
+            assert e.timings()["delta_path"][m] == want, (src, m, e.timings()["delta_path"])
+            ref = oracle.delta(t, A, G, m)
+            dr = ref[_stream_positions(t, idx_s)]
+            sc = np.abs(dr).max(axis=1, keepdims=True) + 1e-30
+            assert (np.abs(d - dr) / sc).max() < 1e-4 * scale_n, (src, m)
+            B, Cv = oracle.bc(t, ref, m, lam)
+            Bg, Cg = e.get_gram(m, t.rows(m))
+            sb = np.abs(B).max(axis=(1, 2), keepdims=True)
+            assert (np.abs(Bg - B) / sb).max() < 2e-4, (src, m)
+            scc = np.abs(Cv).max(axis=1, keepdims=True) + 1e-30
+            assert (np.abs(Cg - Cv) / scc).max() < 2e-4, (src, m)
+        e.close()
 
 
 def test_host_partition_windows_inside_a_heavy_block():
