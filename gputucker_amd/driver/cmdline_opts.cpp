@@ -25,7 +25,8 @@ void CommandLineOptions::PrintHelp() const {
                "  -h [ --help ]                 Display help menu.\n"
                "  -i [ --input ] arg            Input tensor path\n"
                "  -o [ --order ] arg            Order\n"
-               "  -r [ --rank ] arg (=10)       Rank (1..50; BF16 precisions 1..15)\n"
+               "  -r [ --rank ] arg (=10)       Rank (1..50; bf16 1..15, bf16r on the MFMA delta up\n"
+               "                                to 15, and to 32 at order 3; other ranks run FP32)\n"
                "  -g [ --gpus ] arg (=1)        The number of GPUs\n"
                "  --iters arg (=3)              Maximum ALS iterations\n"
                "  --lambda arg (=0.0001)        Ridge regularisation\n"
@@ -38,6 +39,9 @@ void CommandLineOptions::PrintHelp() const {
                "  --resume prefix               Start from the factors and core of an --output run\n"
                "  --fused-fit                   Take each iteration's fit from the row solves\n"
                "  --ref-fit                     Fit in the reference's float arithmetic (serial sums)\n"
+               "  --graph                       Replay each in-core iteration as a captured HIP graph\n"
+               "                                (with --fused-fit)\n"
+               "  --autotune                    Time each mode's chunk sizes and core sources first\n"
                "  --dump-init prefix            Write the initial factors and core, then exit (no GPU)\n"
                "  --cache                       Read/write a binary copy of the input (<input>.gtcoo)\n"
                "  --parse-only                  Parse the input, print its summary, exit (no GPU)\n";
@@ -157,6 +161,10 @@ CommandLineOptions::ReturnStatus CommandLineOptions::Parse(int argc, char *argv[
       _fused_fit = true;
     } else if (a == "--ref-fit") {
       _ref_fit = true;
+    } else if (a == "--graph") {
+      _graph = true;
+    } else if (a == "--autotune") {
+      _autotune = true;
     } else if (a == "--dump-init") {
       if (!need(v)) return OPTS_FAILURE;
       _dump_init = v;

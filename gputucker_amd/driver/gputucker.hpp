@@ -53,6 +53,8 @@ public:
   const std::string &get_resume_prefix() const { return _resume; }
   bool get_fused_fit() const { return _fused_fit; }
   bool get_ref_fit() const { return _ref_fit; }
+  bool get_graph() const { return _graph; }
+  bool get_autotune() const { return _autotune; }
   const std::string &get_dump_init_prefix() const { return _dump_init; }
   void PrintHelp() const;
 
@@ -75,6 +77,8 @@ private:
   std::string _resume;       // start from factors and core written by --output
   bool _fused_fit = false;   // fit from the row solves (gt_iteration)
   bool _ref_fit = false;     // fit in the reference's float arithmetic (gt_opts.ref_fit)
+  bool _graph = false;       // replay in-core iterations as a HIP graph (gt_opts.graph)
+  bool _autotune = false;    // the engine's autotuner at initialisation (gt_opts.autotune)
   std::string _dump_init;    // write the initial factors and core, then stop (no GPU)
 };
 

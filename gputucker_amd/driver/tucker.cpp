@@ -97,6 +97,8 @@ void TuckerDecomposition(tensor_t *t, const CommandLineOptions &o) {
   opts.precision = o.get_precision();
   opts.device_budget = o.get_device_budget();
   opts.ref_fit = o.get_ref_fit() ? 1 : 0;
+  opts.graph = o.get_graph() ? 1 : 0;
+  opts.autotune = o.get_autotune() ? 1 : 0;
   for (size_t a = 0; a < o.get_partition().size(); ++a) opts.partition[a] = o.get_partition()[a];
   printf("\t... Initialize the GPU engine\n");
   if (gt_create(&t->engine, &opts) != GT_OK)

@@ -105,6 +105,9 @@ def test_cache_of_another_order_is_ignored(tmp_path):
 def test_new_flags_are_accepted():
     rc, out = run("-i", README_TNS, "-o", "3", "--parse-only", "--fused-fit", "--resume", "x")
     assert rc == 0, out
+    rc, out = run("-i", README_TNS, "-o", "3", "--parse-only", "--graph", "--autotune",
+                  "--ref-fit")
+    assert rc == 0, out
 
 
 def test_dump_init_follows_glibc_rand_order(tmp_path):
