@@ -186,6 +186,9 @@ contract_slots(int slots, const float *__restrict__ A_L, uint32_t iL,
 // runs, when they cross its edges, to the partial tables of the fix-up.  The
 // tile is kWave rows of pitch RP in LDS, deltas in columns < R and x in
 // column R; `srow` holds the tiles' row ids.
+// nonzeros per step of LaneGram's whole-tile loop
+constexpr int kLaneGramUnroll = 16;
+
 template <int R, int RP> struct LaneGram {
   static constexpr int W = gram_width(R);
   static constexpr int NE = gram_entries(R);
@@ -244,6 +247,32 @@ template <int R, int RP> struct LaneGram {
 
   // Adds tile [tb, tb + cnt); tile and srow must be visible to the whole wave.
   __device__ void add_tile(const float *tile, const uint32_t *srow, uint64_t tb, int cnt) {
+    // A whole tile that continues the current run (the common case inside
+    // long rows) needs no row checks: each entry's two columns are read
+    // through per-lane pointers with the nonzero's offset as an immediate,
+    // so the loop spends no arithmetic on addresses.  Same products, same
+    // order of the sums as the general loop below.
+    if (cnt == kWave && __ballot(srow[lane] != cur) == 0) {
+      const float *bp[E], *bq[E];
+#pragma unroll
+      for (int qi = 0; qi < E; ++qi) {
+        bp[qi] = tile + (off[qi] & 0xFFFFu);
+        bq[qi] = tile + (off[qi] >> 16);
+      }
+      for (int j0 = 0; j0 < kWave; j0 += kLaneGramUnroll) {
+#pragma unroll
+        for (int j = 0; j < kLaneGramUnroll; ++j)
+#pragma unroll
+          for (int qi = 0; qi < E; ++qi)
+            acc[qi] += (double)bp[qi][j * RP] * (double)bq[qi][j * RP];
+#pragma unroll
+        for (int qi = 0; qi < E; ++qi) {
+          bp[qi] += kLaneGramUnroll * RP;
+          bq[qi] += kLaneGramUnroll * RP;
+        }
+      }
+      return;
+    }
     for (int j = 0; j < cnt; ++j) {
       const uint32_t row = __builtin_amdgcn_readfirstlane(srow[j]);
       if (row != cur) {
