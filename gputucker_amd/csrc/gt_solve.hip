@@ -5,7 +5,8 @@
 // B_i is symmetric positive definite (lambda > 0), so here each row is solved
 // by an FP64 Cholesky factorisation and two triangular solves on the GPU:
 // one lane per row for R <= 16 (the factor lives in registers), one wave per
-// row above that (the factor lives in LDS).  Rows without nonzeros have
+// row above that (lane i holds row i of the factor in registers up to
+// R = 36, in LDS above).  Rows without nonzeros have
 // B = lambda I and C = 0, so their solution is 0 (the reference's result too).
 //
 // With fit_partials set, each solve also yields the row's share of the
@@ -172,9 +173,113 @@ __global__ void __launch_bounds__(64) k_solve_lane(SolveArgs a) {
   }
 }
 
-// One wave per row; lane i owns row i of the factor (R <= 64).
+// Largest rank whose wave solve keeps the factor's rows in registers (at
+// most 168 VGPRs, three waves per SIMD); above it the factor lives in LDS.
+constexpr int kSolveRegMax = 36;
+
+// A double read from lane `l` of the wave (l wave-uniform), as an SGPR pair.
+__device__ inline double readlane_f64(double v, int l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Orders the wave's LDS writes before its later reads and back.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wave per row (16 < R <= 64).  Lane i < R holds row i of B in
+// registers and turns it into row i of the Cholesky factor L, right-looking:
+// at step j the diagonal comes from lane j, each lane below scales its
+// L(i, j), column j goes through LDS to every lane, and each lane updates
+// its own row (M(i, k) -= L(i, j) L(k, j), k > j).  Every step is unrolled,
+// so the row stays in registers.  The forward solve L y = c then runs on the
+// rows in registers (y_k from lane k to all), and L goes to LDS for the
+// backward solve L^T a = y (row k of L read by the lanes of its columns) and
+// the fit partial.  Lanes at and past R shadow row R - 1; their results are
+// never used.  Above the diagonal the registers hold values no step reads.
 template <int R>
 __global__ void __launch_bounds__(64) k_solve_wave(SolveArgs a) {
+  constexpr int W = gram_width(R);
+  constexpr int LP = R + 1; // pitch of L in LDS (doubles)
+  __shared__ double colj[R];
+  __shared__ double Lm[R * LP];
+  const uint64_t row = blockIdx.x;
+  const int lane = threadIdx.x;
+  float *out = a.A + row * row_stride(R);
+  if (a.row_ptr[row + 1] == a.row_ptr[row]) {
+    if (lane < R) out[lane] = 0.f;
+    if (a.fit_partials && lane == 0) a.fit_partials[row] = 0.0;
+    return;
+  }
+  const double *g = a.gram + row * W;
+  const int li = lane < R ? lane : R - 1;
+  double m[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const double v = g[k <= li ? up_index(R, k, li) : 0];
+    m[k] = k < li ? v : (k == li ? v + a.lambda : 0.0);
+  }
+  const double ci = g[R * (R + 1) / 2 + li];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const double d = sqrt(readlane_f64(m[j], j));
+    m[j] = li == j ? d : (li > j ? m[j] / d : m[j]);
+    colj[li] = m[j];
+    wave_lds_sync();
+#pragma unroll
+    for (int k = j + 1; k < R; ++k) m[k] -= m[j] * colj[k];
+    wave_lds_sync(); // every read of column j before column j + 1 is written
+  }
+  // forward: y_k = (c_k - sum_{i<k} L(k, i) y_i) / L(k, k), lane k's c
+  // holding the bracket when step k comes
+  double c = ci, y = 0.0;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const double yk = readlane_f64(c, k) / readlane_f64(m[k], k);
+    if (li == k) y = yk;
+    c -= m[k] * yk;
+  }
+  // L to LDS, row i at i * LP
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+    if (k <= li) Lm[li * LP + k] = m[k];
+  wave_lds_sync();
+  // backward: a_k = (y_k - sum_{i>k} L(i, k) a_i) / L(k, k), descending k;
+  // lane i < k subtracts L(k, i) a_k from its y.  With the fit partial, lane i
+  // also sums v_i = (L^T a)_i = sum_{k >= i} L(k, i) a_k, a as stored (FP32)
+  double v = 0.0;
+  float af = 0.f;
+#pragma unroll
+  for (int k = R - 1; k >= 0; --k) {
+    const double ak = readlane_f64(y, k) / Lm[k * LP + k];
+    const float akf = (float)ak;
+    if (li == k) af = akf;
+    const double lki = Lm[k * LP + (li <= k ? li : k)];
+    if (li < k) y -= lki * ak;
+    if (li <= k) v += lki * (double)akf;
+  }
+  if (lane < R) out[lane] = af;
+  if (a.fit_partials) { // a^T (L L^T) a - lambda |a|^2 - 2 a^T c
+    double contrib = 0.0;
+    if (lane < R) {
+      const double ai = (double)af;
+      contrib = v * v - a.lambda * ai * ai - 2.0 * ai * ci;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
+    if (lane == 0) a.fit_partials[row] = contrib;
+  }
+}
+
+// One wave per row with the factor in LDS (R > kSolveRegMax): lane i owns
+// row i; the register form below would need more than 168 VGPRs.
+template <int R>
+__global__ void __launch_bounds__(64) k_solve_wave_lds(SolveArgs a) {
   constexpr int W = gram_width(R);
   __shared__ double M[R][R + 1];
   __shared__ double y[R];
@@ -237,8 +342,11 @@ template <int R> uint64_t solve_impl(const SolveArgs &a, hipStream_t s) {
     const uint64_t grid = (a.rows + 63) / 64;
     hipLaunchKernelGGL(k_solve_lane<R>, dim3((unsigned)grid), dim3(64), 0, s, a);
     return grid;
-  } else {
+  } else if constexpr (R <= kSolveRegMax) {
     hipLaunchKernelGGL(k_solve_wave<R>, dim3((unsigned)a.rows), dim3(64), 0, s, a);
+    return a.rows;
+  } else {
+    hipLaunchKernelGGL(k_solve_wave_lds<R>, dim3((unsigned)a.rows), dim3(64), 0, s, a);
     return a.rows;
   }
 }
