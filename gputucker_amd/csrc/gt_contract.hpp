@@ -9,6 +9,20 @@ namespace gt {
 
 constexpr int kSlots = kMaxOrder - 2;
 
+// x[l] + x[l ^ 32] and x[l] + x[l ^ 16] through the gfx950 lane swaps
+// (v_permlane32_swap / v_permlane16_swap: VALU, no LDS round trip).  Each
+// returns the same sum in both lanes of a pair (IEEE addition commutes).
+__device__ inline __attribute__((always_inline)) float add_xor32(float x) {
+  const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false,
+                                                  false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+__device__ inline __attribute__((always_inline)) float add_xor16(float x) {
+  const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false,
+                                                  false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+
 __device__ inline __attribute__((always_inline)) void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

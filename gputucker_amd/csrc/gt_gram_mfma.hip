@@ -739,7 +739,7 @@ k_gram_bf16_o5(GramArgs a, int wpb) {
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) dacc[mb][i] += __shfl_xor(dacc[mb][i], 32);
+      for (int i = 0; i < 4; ++i) dacc[mb][i] = add_xor32(dacc[mb][i]);
     // the next tile's gathers and the stream loads of the tile after it
     gather();
     load_idx(tb + 2 * kWave);
@@ -949,8 +949,8 @@ k_gram_bf16_ob(GramArgs a, int wpb) {
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        dacc[mb][r] += __shfl_xor(dacc[mb][r], 16);
-        dacc[mb][r] += __shfl_xor(dacc[mb][r], 32);
+        dacc[mb][r] = add_xor16(dacc[mb][r]);
+        dacc[mb][r] = add_xor32(dacc[mb][r]);
       }
     wave_sync(); // the s block's reads are done before the tile overwrites it
     // lane (q, g) writes the deltas of nonzero 16 g + q = lane, then x over
