@@ -25,8 +25,9 @@ void CommandLineOptions::PrintHelp() const {
                "  -h [ --help ]                 Display help menu.\n"
                "  -i [ --input ] arg            Input tensor path\n"
                "  -o [ --order ] arg            Order\n"
-               "  -r [ --rank ] arg (=10)       Rank (1..50; bf16 1..15, bf16r on the MFMA delta up\n"
-               "                                to 15, and to 32 at order 3; other ranks run FP32)\n"
+               "  -r [ --rank ] arg (=10)       Rank (1..50; bf16 and bf16r run the MFMA delta at\n"
+               "                                order 3 up to R = 15 (bf16r to 32), order 4 up to\n"
+               "                                10, order 5 up to 8; elsewhere the FP32 delta)\n"
                "  -g [ --gpus ] arg (=1)        The number of GPUs\n"
                "  --iters arg (=3)              Maximum ALS iterations\n"
                "  --lambda arg (=0.0001)        Ridge regularisation\n"

@@ -145,12 +145,18 @@ template <> bool ParseFromFile<tensor_t>(const std::string &file_name, tensor_t 
   for (int a = 0; a < order; ++a) t->coo_indices[a].assign(max_lines, 0);
   t->coo_values.assign(max_lines, 0.f);
   std::vector<uint8_t> has_val(max_lines, 1), keep(max_lines, 0);
+  // per-thread results, merged after the parallel region (each thread keeps
+  // its running max / min in locals: shared per-thread slots next to each
+  // other would put every token's update on a cache line the others write)
   std::vector<std::vector<uint32_t>> tmax(T, std::vector<uint32_t>(order, 0));
   std::vector<uint64_t> tmin(T, UINT64_MAX);
   std::vector<int> bad(T, 0);
 #pragma omp parallel num_threads(T)
   {
     const int tid = omp_get_thread_num();
+    uint32_t lmax[constants::kMaxOrder] = {0};
+    uint64_t lmin = UINT64_MAX;
+    int lbad = 0;
     uint64_t li = lines[tid];
     size_t p = start[tid];
     const size_t end = start[tid + 1];
@@ -169,13 +175,20 @@ template <> bool ParseFromFile<tensor_t>(const std::string &file_name, tensor_t 
         if (axis < order) {
           unsigned long long v = 0;
           std::from_chars(buf + s, buf + e, v); // strtoull: leading digits, else 0
-          tmax[tid][axis] = std::max<uint32_t>(tmax[tid][axis], (uint32_t)v);
-          tmin[tid] = std::min<uint64_t>(tmin[tid], v);
+          lmax[axis] = std::max<uint32_t>(lmax[axis], (uint32_t)v);
+          lmin = std::min<uint64_t>(lmin, v);
           t->coo_indices[axis][li] = (uint32_t)v - 1;
           ++axis;
         } else {
-          std::string tok(buf + s, e - s);
-          t->coo_values[li] = (float)std::strtod(tok.c_str(), nullptr); // std::stod
+          // std::stod, then value_t: from_chars rounds like strtod; forms it
+          // does not take whole ('+', hex, inf/nan spellings) go to strtod
+          double d = 0.0;
+          const auto r = std::from_chars(buf + s, buf + e, d);
+          if (r.ec != std::errc() || r.ptr != buf + e) {
+            std::string tok(buf + s, e - s);
+            d = std::strtod(tok.c_str(), nullptr);
+          }
+          t->coo_values[li] = (float)d;
           valued = true;
           break;
         }
@@ -184,7 +197,7 @@ template <> bool ParseFromFile<tensor_t>(const std::string &file_name, tensor_t 
       if (axis == 0) {
         keep[li] = 0; // blank line
       } else if (axis < order) {
-        bad[tid] = 1;
+        lbad = 1;
       } else {
         keep[li] = 1;
         has_val[li] = valued;
@@ -192,6 +205,9 @@ template <> bool ParseFromFile<tensor_t>(const std::string &file_name, tensor_t 
       ++li;
       p = q + 1;
     }
+    for (int ax = 0; ax < order; ++ax) tmax[tid][ax] = lmax[ax];
+    tmin[tid] = lmin;
+    bad[tid] = lbad;
   }
   if (len) munmap((void *)buf, len);
   close(fd);

@@ -146,3 +146,16 @@ def test_dump_init_follows_glibc_rand_order(tmp_path):
     want = np.array([frand() for _ in range(R ** 3)], np.float32).reshape(-1, R)
     got = np.loadtxt(f"{prefix}_core.txt", ndmin=2).astype(np.float32)
     assert np.array_equal(got, want)
+
+
+def test_value_spellings_read_like_stod(tmp_path):
+    """Values are read as std::stod reads them (tensor_manager.tpp:150-155):
+    signs, exponents, hex floats and trailing text after the number."""
+    p = tmp_path / "spell.tns"
+    p.write_text("1 1 1 +2.5\n2 1 1 0x1p1\n1 2 1 1e-3\n2 2 1 -3\n1 1 2 7.25abc\n")
+    rc, out = run("-i", str(p), "-o", "3", "--parse-only")
+    assert rc == 0, out
+    order, nnz, dims, ss = summary(out)
+    assert (nnz, dims) == (5, [2, 2, 2])
+    want = 2.5 ** 2 + 2.0 ** 2 + 0.001 ** 2 + 9.0 + 7.25 ** 2
+    assert abs(ss - want) < 1e-6 * want
