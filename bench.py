@@ -73,11 +73,11 @@ _PATH_KERNEL = {"fp32": "k_gram_valu<{R}>", "fp32-wide": "k_gram_fp32<{R}>",
                 "bf16": "k_gram_bf16<{R}, 1>", "bf16r": "k_gram_bf16<{R}, 2>",
                 "bf16r-l2": "k_gram_bf16_o5<{R}>", "bf16r-wide": "k_gram_bf16w<{R}>",
                 "bf16r-ob": "k_gram_bf16_ob<{R}, {NS}, true>",
-                "bf16r-ob-l2": "k_gram_bf16_ob<{R}, {NS}, false>"}
+                "bf16r-ob-l2": "k_gram_bf16_ob<{R}, {NS}, false>", "bf16r-kr": "k_gram_bf16_kr<{R}>"}
 # the arithmetic each path's delta runs in
 _PATH_DTYPE = {"fp32": "fp32", "fp32-wide": "fp32", "bf16": "bf16", "bf16r": "bf16r",
                "bf16r-l2": "bf16r", "bf16r-wide": "bf16r", "bf16r-ob": "bf16r",
-               "bf16r-ob-l2": "bf16r"}
+               "bf16r-ob-l2": "bf16r", "bf16r-kr": "bf16r"}
 
 
 def committed_counters(workload, kernel):
@@ -120,7 +120,8 @@ def kernel_bytes_per_nnz(N, R, path):
     indices and the value (no row-grouping entry: the stream is row-sorted),
     the split BF16 copy of the last other factor (16 words per row) on the
     MFMA paths or its FP32 row, and the other factors' FP32 rows."""
-    a_l = 128 if path == "bf16r-wide" else (64 if path.startswith("bf16") else 4 * R)
+    a_l = (128 if path == "bf16r-wide" else
+           (4 * R if path == "bf16r-kr" else (64 if path.startswith("bf16") else 4 * R)))
     return 4 * N + 4 + a_l + (N - 2) * 4 * R
 
 

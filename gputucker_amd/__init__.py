@@ -22,7 +22,7 @@ MAX_ORDER = 8
 PRECISIONS = {"fp32": 0, "bf16": 1, "bf16r": 2}
 # gt_delta_path: the delta + Gram kernel a mode update ran
 DELTA_PATHS = {0: "none", 1: "fp32", 2: "fp32-wide", 3: "bf16", 4: "bf16r", 5: "bf16r-l2",
-               6: "bf16r-wide", 7: "bf16r-ob", 8: "bf16r-ob-l2"}
+               6: "bf16r-wide", 7: "bf16r-ob", 8: "bf16r-ob-l2", 9: "bf16r-kr"}
 STATUS = {0: "GT_OK", 1: "GT_ERR_ARG", 2: "GT_ERR_HIP", 3: "GT_ERR_STATE",
           4: "GT_ERR_UNSUPPORTED", 5: "GT_ERR_MEMORY", 6: "GT_ERR_PARTITION"}
 

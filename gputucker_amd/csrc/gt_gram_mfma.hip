@@ -971,6 +971,203 @@ k_gram_bf16_ob(GramArgs a, int wpb) {
   gram.finish();
 }
 
+// ------------------------------------------------------------ BF16, order 4, Kronecker
+// The refined BF16 delta of an order-4 tensor at R = 4 or 8 with two modes
+// contracted on the matrix cores (table layout in gt_internal.hpp).  With
+// outer slots 0 and 1 (o0, o1) and L the last other mode,
+//   stage 1 (MFMA): V[m][(o0, r)] = sum_{o1, jL} G[o0][o1][jL][r] k_m[o1 R + jL],
+//                   k_m[o1 R + jL] = A_O1[i_O1(m)][o1] * A_L[i_L(m)][jL] (FP32),
+//   stage 2 (VALU): delta[m][r] = sum_{o0} A_O0[i_O0(m)][o0] V[m][(o0, r)].
+// k is split as k_hi + k_lo (BF16) per lane on the fly; one MFMA per 16
+// nonzeros covers 16 output rows (o0, r) and 16 (HL) or 32 (HH) entries of k.
+// Stage 2 is R FMAs per output, against O R = R^3 for the o-block kernel.
+// Each lane gathers its own nonzero's three rows; they go to this wave's LDS
+// (outer rows permuted so that the values a lane group needs are
+// contiguous) and each lane reads the four nonzeros of its MFMA columns.
+// The table sits in LDS for a workgroup of 12 waves.
+template <int R> constexpr int kr_rc() { return 16 / R; } // o = RC * block + lane-group part
+template <int R> constexpr int kr_sigma(int o) {          // LDS position of outer value o
+  return (o % kr_rc<R>()) * kr_nhl(R) + o / kr_rc<R>();
+}
+template <int R> constexpr size_t kr_lds_floats_per_wave() {
+  // the three rows of 64 nonzeros, or the delta tile; then the row ids
+  return (size_t)((3 * R * kWave > 16 * kDP) ? 3 * R * kWave : 16 * kDP) + kWave;
+}
+
+template <int R>
+__global__ void __launch_bounds__(64 * kTableWaves) __attribute__((amdgpu_waves_per_eu(3)))
+k_gram_bf16_kr(GramArgs a, int wpb) {
+  static_assert(R == 4 || R == 8, "the Kronecker kernel covers order 4 at R = 4 and 8");
+  constexpr int NHL = kr_nhl(R), NHH = kr_nhh(R), NMF = NHL + NHH, NKB = kr_nkb(R);
+  constexpr int NP = (NHL + 1) / 2; // 8-word operand blocks per block of 16 nonzeros
+  constexpr int T = ChunkGram<R>::T;
+  constexpr int RW = 3 * R; // floats per nonzero in the row area: A_L, A_O1, A_O0
+  constexpr size_t kBuf = kr_lds_floats_per_wave<R>() - kWave;
+  constexpr size_t kTabFloats = gb_kr_words(R) / 2;
+  extern __shared__ __align__(16) float dyn[];
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
+  {
+    const uint4 *src = reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(a.Gb) +
+                                                       gb_kr_offset(R, 4, R * R));
+    uint4 *dst = reinterpret_cast<uint4 *>(dyn);
+    for (int i = threadIdx.x; i < NKB * NMF * kWave; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+  }
+  const s8 *tab = reinterpret_cast<const s8 *>(dyn) + lane;
+  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
+  if (c >= a.chunk_end) return;
+  float *buf = dyn + kTabFloats + (size_t)w * kr_lds_floats_per_wave<R>();
+  float *tile = buf; // after stage 2; the row area before it
+  uint32_t *s_rowv = reinterpret_cast<uint32_t *>(buf + kBuf);
+  const uint64_t cb = c * a.chunk;
+  const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
+  const int q = lane & 15, g = lane >> 4;
+  const int jl0 = (4 * g) % R;         // this lane group's first jL
+  const int oc = (4 * g) / R;          // its part of o0 and o1
+  ChunkGram<R> gram(a, c, cb, ce, lane);
+
+  uint32_t n_row, n_iL, n_io0, n_io1;
+  float n_x;
+  auto load_idx = [&](uint64_t tb) {
+    const uint64_t e = tb + lane;
+    const uint64_t ee = e < ce ? e : cb;
+    n_row = a.idx_row[ee];
+    n_iL = a.idx_L[ee];
+    n_io0 = a.idx_O[0][ee];
+    n_io1 = a.idx_O[1][ee];
+    n_x = e < ce ? a.st.val[ee] : 0.f;
+  };
+  float rL[R], rO0[R], rO1[R]; // this lane's nonzero's rows (the next tile's, in flight)
+  auto gather = [&]() {
+    load_row<R>(a.A_L, n_iL, rL);
+    load_row<R>(a.A_O[0], n_io0, rO0);
+    load_row<R>(a.A_O[1], n_io1, rO1);
+  };
+  load_idx(cb);
+  uint32_t c_row = n_row;
+  float c_x = n_x;
+  gather();
+  load_idx(cb + kWave);
+
+  for (uint64_t tb = cb; tb < ce; tb += kWave) {
+    const bool valid = tb + lane < ce;
+    const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
+    const uint32_t row = c_row;
+    const float x = c_x;
+    c_row = n_row;
+    c_x = n_x;
+    s_rowv[lane] = valid ? row : 0xFFFFFFFFu;
+    {
+      float *my = buf + lane * RW;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        my[j] = rL[j];
+        my[R + kr_sigma<R>(j)] = rO1[j];
+        my[2 * R + kr_sigma<R>(j)] = rO0[j];
+      }
+    }
+    wave_sync();
+    // the factor-side words of the four blocks of 16 nonzeros: per pair p of
+    // HL MFMAs, [lo(k[2p]), hi(k[2p]), hi(k[2p + 1]), lo(k[2p + 1])]
+    gt_u4 V[4][NP][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const float *rm = buf + (mb * 16 + q) * RW;
+      const gt_f4 aL = *reinterpret_cast<const gt_f4 *>(rm + jl0);
+      float ao[NHL];
+#pragma unroll
+      for (int mu = 0; mu < NHL; ++mu) ao[mu] = rm[R + oc * NHL + mu];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        uint32_t hw[2][2], lw[2][2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int mu = 2 * p + h;
+          if (mu < NHL) {
+            float k[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) k[i] = ao[mu] * aL[i];
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+              const uint32_t hh = pk_bf16(k[2 * d], k[2 * d + 1]);
+              hw[h][d] = hh;
+              lw[h][d] = pk_bf16(k[2 * d] - bf16_low_f(hh), k[2 * d + 1] - bf16_high_f(hh));
+            }
+          } else {
+            hw[h][0] = hw[h][1] = lw[h][0] = lw[h][1] = 0u;
+          }
+        }
+        V[mb][p][0] = gt_u4{lw[0][0], lw[0][1], hw[0][0], hw[0][1]};
+        V[mb][p][1] = gt_u4{hw[1][0], hw[1][1], lw[1][0], lw[1][1]};
+      }
+    }
+    float dacc[4][4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
+#pragma unroll
+    for (int kb = 0; kb < NKB; ++kb) {
+      f4 D[4];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) D[mb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int mu = 0; mu < NMF; ++mu) {
+        const b8 t = __builtin_bit_cast(b8, tab[(kb * NMF + mu) * kWave]);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+          b8 bop;
+          if (mu < NHL) { // HL: block p = mu / 2, first or second half
+            const gt_u4 v = V[mb][mu / 2][mu & 1];
+            bop = __builtin_bit_cast(b8, v);
+          } else { // HH: the middle of block p = mu - NHL
+            const int p = mu - NHL;
+            const gt_u4 v = gt_u4{V[mb][p][0][2], V[mb][p][0][3], V[mb][p][1][0], V[mb][p][1][1]};
+            bop = __builtin_bit_cast(b8, v);
+          }
+          D[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(t, bop, D[mb], 0, 0, 0);
+        }
+      }
+      // rows 16 kb + 4g + i are (o0, r) = (RC kb + oc, jl0 + i): scale by A_O0[o0]
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const float wv = buf[(mb * 16 + q) * RW + 2 * R + oc * NKB + kb];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(D[mb][i], wv, dacc[mb][i]);
+      }
+    }
+    // the next tile's rows and the stream loads of the tile after it (issued
+    // after the stage loop, whose operands fill the registers)
+    gather();
+    load_idx(tb + 2 * kWave);
+    // lane groups with the same jl0 hold the same columns for other o0: add
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if constexpr (R == 4) dacc[mb][i] = add_xor16(dacc[mb][i]);
+        dacc[mb][i] = add_xor32(dacc[mb][i]);
+      }
+    wave_sync(); // every read of the row area before the tile overwrites it
+    // lane (q, g < R / 4) holds delta[16mb + q][4g + i]; then x over column R
+    // and zeros up to the Gram's last column tile
+    if (g < R / 4) {
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) tile[(4 * g + i) * kDP + mb * 16 + q] = dacc[mb][i];
+    }
+    tile[R * kDP + lane] = x;
+#pragma unroll
+    for (int r = R + 1; r < 4 * T; ++r) tile[r * kDP + lane] = 0.f;
+    wave_sync();
+    dump_tile<R>(a, tile, tb, cnt, lane);
+    gram.add_tile(tile, s_rowv, tb, cnt);
+    wave_sync();
+  }
+  gram.finish();
+}
+
 // ------------------------------------------------------------ split factor
 // Row i of the split copy holds, for each lane group g, the 16 bytes of BF16
 // words that group's K=32 slots take (bf16r_plan), so the gather of one
@@ -1047,6 +1244,25 @@ template <int R, int NS> int launch_ob(const GramArgs &a, hipStream_t st, bool a
   }
 }
 
+// Launches k_gram_bf16_kr over the chunks of `a` (if any); -1 where it is
+// not built for R or the table and twelve waves do not fit in LDS.
+template <int R> int launch_kr(const GramArgs &a, hipStream_t st, bool any) {
+  if constexpr (!bf16r_kr_table(R, 4)) {
+    return -1;
+  } else {
+    const size_t lds = gb_kr_words(R) * sizeof(uint16_t) +
+                       (size_t)kTableWaves * kr_lds_floats_per_wave<R>() * sizeof(float);
+    if (lds > lds_limit()) return -1;
+    if (any) {
+      const unsigned grid =
+          (unsigned)((a.chunk_end - a.chunk_begin + kTableWaves - 1) / kTableWaves);
+      hipLaunchKernelGGL((k_gram_bf16_kr<R>), dim3(grid), dim3(64 * kTableWaves), lds, st, a,
+                         kTableWaves);
+    }
+    return GT_DELTA_BF16R_KR;
+  }
+}
+
 // Picks the delta + Gram kernel for rank R and the requested delta kind, and
 // launches it over the chunks of `a` (if any); returns the gt_delta_path
 // that ran.  A BF16 request falls back to the FP32 delta only where no MFMA
@@ -1063,6 +1279,19 @@ template <int R, int KIND> int launch_impl(const GramArgs &a, int O, hipStream_t
       // orders 4-5: o-blocks up to kObMaxO outer indices, then (order 5,
       // R <= 8) pairs of outer indices
       if (a.As_L != nullptr && (a.slots == 2 || a.slots == 3) && O == cpow(R, a.slots)) {
+        // order 4 at R = 4, 8: two modes on the matrix cores; the other
+        // source (a.alt_source) is the o-block kernel with its table in LDS
+        if (a.slots == 2 && bf16r_kr_table(R, 4)) {
+          if (!a.alt_source) {
+            const int p = launch_kr<R>(a, st, any);
+            if (p >= 0) return p;
+          } else {
+            GramArgs b = a;
+            b.alt_source = 0;
+            const int p = launch_ob<R, 2>(b, st, any);
+            if (p >= 0) return p;
+          }
+        }
         const int p = a.slots == 2 ? launch_ob<R, 2>(a, st, any) : launch_ob<R, 3>(a, st, any);
         if (p >= 0) return p;
         if (a.slots == 3 && launch_o5<R>(a, st, any)) return GT_DELTA_BF16R_L2;

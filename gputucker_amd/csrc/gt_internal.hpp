@@ -247,6 +247,31 @@ __host__ __device__ constexpr size_t gb_ob_offset(int R, int O) {
 __host__ __device__ constexpr size_t gb_ob_words(int R, int O) {
   return (size_t)gb_oblocks(O) * R * kWave * 8;
 }
+// Order 4 at R = 4 and 8 gets a last table for the two-mode refined BF16
+// delta, the Kronecker table ([kb][mu][lane][8]).  With outer slots 0 and 1
+// (values o0, o1) and e = o1 * R + jL, stage 1 contracts (o1, jL) on the
+// matrix cores: output row rho = 16 kb + l % 16 of block kb is (o0, r) =
+// (rho / R, rho % R), and the K slots of lane group g take the entries
+// e = 16 mu + 4g + i (i < 4) of the factor-side operand k[e] = A_O1[o1] *
+// A_L[jL].  MFMAs mu < kr_nhl carry k_hi.G_hi + k_lo.G_hi (factor words
+// [lo(k[4 entries]), hi(...)] for even mu, [hi, lo] for odd mu; core words
+// G_hi twice), MFMAs kr_nhl + nu carry k_hi.G_lo over the entries of
+// mu = 2 nu and 2 nu + 1 (factor words [hi(k[2 nu]), hi(k[2 nu + 1])],
+// core words G_lo).  Even/odd orders let one 8-word register block per pair
+// of mu hold both HL operands and the HH operand between them.  R divides
+// 16, so a lane's jL = 4g % R + i and o1 = (16 mu + 4g) / R separate.
+__host__ __device__ constexpr bool bf16r_kr_table(int R, int N) {
+  return N == 4 && (R == 4 || R == 8);
+}
+__host__ __device__ constexpr int kr_nhl(int R) { return R * R / 16; }
+__host__ __device__ constexpr int kr_nhh(int R) { return (R * R + 31) / 32; }
+__host__ __device__ constexpr int kr_nkb(int R) { return R * R / 16; }
+__host__ __device__ constexpr size_t gb_kr_offset(int R, int N, int O) {
+  return gb_ob_offset(R, O) + (bf16r_ob_table(R, N, O) ? gb_ob_words(R, O) : 0);
+}
+__host__ __device__ constexpr size_t gb_kr_words(int R) {
+  return (size_t)kr_nkb(R) * (kr_nhl(R) + kr_nhh(R)) * kWave * 8;
+}
 // Returns the gt_delta_path it took.
 int launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);
 // Split BF16 copy of a factor for the MFMA delta (R <= 15): 16 uint32 words

@@ -272,13 +272,15 @@ def test_error_paths():
                                       (3, 13, [2, 1, 2]), (3, 15, [1, 1, 1]), (3, 7, [1, 2, 1]),
                                       (2, 13, [2, 1]), (4, 7, [1, 2, 1, 1]), (4, 9, [1, 1, 2, 1]),
                                       (5, 4, [1, 1, 1, 2, 1]), (5, 5, [1, 1, 1, 2, 1]),
+                                      (4, 8, [1, 1, 2, 1]), (4, 4, [2, 1, 1, 1]),
                                       (3, 20, [2, 1, 2]), (3, 32, [1, 1, 2]), (3, 24, [1, 2, 1])])
 def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
     """BF16 MFMA delta (plain and with the split-BF16 refinement; above
     R = 10 the refinement takes a second MFMA per tile; orders 4 and 5 run
     the refined delta on the o-block kernel, 16 outer indices per MFMA, at
     R = 5, 7 and 9 of order 4 and R = 4, 5 of order 5 with partial blocks
-    or none; plain BF16 there takes the order-3 kernel's loop over outer
+    or none, and at order 4 with R = 4 and 8 the Kronecker kernel, two modes
+    per MFMA; plain BF16 there takes the order-3 kernel's loop over outer
     indices) and the FP32 VALU delta (two tiles per pass) feeding the MFMA
     Gram, against the FP64 oracle's B and C."""
     if R > 15 and prec == "bf16":
@@ -601,7 +603,8 @@ def test_autotune_keeps_parity(prec, N, R, part):
 
 @pytest.mark.parametrize("N,R,part,paths", [
     (4, 5, [1, 2, 1, 1], ("bf16r-ob", "bf16r-ob-l2")),
-    (4, 8, [1, 1, 2, 1], ("bf16r-ob", "bf16r-ob-l2")),
+    (4, 8, [1, 1, 2, 1], ("bf16r-kr", "bf16r-ob")),
+    (4, 4, [2, 1, 1, 1], ("bf16r-kr", "bf16r-ob")),
     (4, 10, [1, 1, 1, 2], ("bf16r-ob", "bf16r-ob-l2")),
     (5, 5, [1, 1, 1, 2, 1], ("bf16r-ob", "bf16r-ob-l2")),
     (5, 2, [1, 1, 1, 1, 2], ("bf16r-ob", "bf16r-ob-l2")),
@@ -609,7 +612,8 @@ def test_autotune_keeps_parity(prec, N, R, part):
 def test_core_source_variants(N, R, part, paths):
     """Both sources of the refined BF16 core operand (core_source 0 and 1):
     the o-block kernel (orders 4-5 with R^(N-2) <= 128, the o-block table in
-    LDS or from L2, with partial last blocks at R = 5 and 10) and, above
+    LDS or from L2, with partial last blocks at R = 5 and 10; at order 4 with
+    R = 4 and 8 the Kronecker kernel and the o-block kernel) and, above
     that, the pair kernel (its pair table, 83 KiB at order 5 and R = 6, does
     not go in LDS beside twelve waves: L2 either way).  Each one's fused
     deltas against the FP64 oracle's (delta.cuh:62-90), and its B and C
