@@ -1067,73 +1067,81 @@ k_gram_bf16_kr(GramArgs a, int wpb) {
       }
     }
     wave_sync();
-    // the factor-side words of the four blocks of 16 nonzeros: per pair p of
-    // HL MFMAs, [lo(k[2p]), hi(k[2p]), hi(k[2p + 1]), lo(k[2p + 1])]
-    gt_u4 V[4][NP][2];
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-      const float *rm = buf + (mb * 16 + q) * RW;
-      const gt_f4 aL = *reinterpret_cast<const gt_f4 *>(rm + jl0);
-      float ao[NHL];
-#pragma unroll
-      for (int mu = 0; mu < NHL; ++mu) ao[mu] = rm[R + oc * NHL + mu];
-#pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        uint32_t hw[2][2], lw[2][2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int mu = 2 * p + h;
-          if (mu < NHL) {
-            float k[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) k[i] = ao[mu] * aL[i];
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {
-              const uint32_t hh = pk_bf16(k[2 * d], k[2 * d + 1]);
-              hw[h][d] = hh;
-              lw[h][d] = pk_bf16(k[2 * d] - bf16_low_f(hh), k[2 * d + 1] - bf16_high_f(hh));
-            }
-          } else {
-            hw[h][0] = hw[h][1] = lw[h][0] = lw[h][1] = 0u;
-          }
-        }
-        V[mb][p][0] = gt_u4{lw[0][0], lw[0][1], hw[0][0], hw[0][1]};
-        V[mb][p][1] = gt_u4{hw[1][0], hw[1][1], lw[1][0], lw[1][1]};
-      }
-    }
+    // Two halves of 32 nonzeros (blocks mb = 2h, 2h + 1), so the operands of
+    // one half fill the registers: per pair p of HL MFMAs the factor-side
+    // words are [lo(k[2p]), hi(k[2p]), hi(k[2p + 1]), lo(k[2p + 1])]
     float dacc[4][4];
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
+    for (int h = 0; h < 2; ++h) {
+      // no instruction of one half moves into the other: the scheduler would
+      // hoist the second half's operands over the first half's MFMAs, and
+      // both halves' registers would be live at once
+      __builtin_amdgcn_sched_barrier(0);
+      gt_u4 V[2][NP][2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
+      for (int b = 0; b < 2; ++b) {
+        const float *rm = buf + ((2 * h + b) * 16 + q) * RW;
+        const gt_f4 aL = *reinterpret_cast<const gt_f4 *>(rm + jl0);
+        float ao[NHL];
 #pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) {
-      f4 D[4];
+        for (int mu = 0; mu < NHL; ++mu) ao[mu] = rm[R + oc * NHL + mu];
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb) D[mb] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int p = 0; p < NP; ++p) {
+          uint32_t hw[2][2], lw[2][2];
 #pragma unroll
-      for (int mu = 0; mu < NMF; ++mu) {
-        const b8 t = __builtin_bit_cast(b8, tab[(kb * NMF + mu) * kWave]);
+          for (int e = 0; e < 2; ++e) {
+            const int mu = 2 * p + e;
+            if (mu < NHL) {
+              float k[4];
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-          b8 bop;
-          if (mu < NHL) { // HL: block p = mu / 2, first or second half
-            const gt_u4 v = V[mb][mu / 2][mu & 1];
-            bop = __builtin_bit_cast(b8, v);
-          } else { // HH: the middle of block p = mu - NHL
-            const int p = mu - NHL;
-            const gt_u4 v = gt_u4{V[mb][p][0][2], V[mb][p][0][3], V[mb][p][1][0], V[mb][p][1][1]};
-            bop = __builtin_bit_cast(b8, v);
+              for (int i = 0; i < 4; ++i) k[i] = ao[mu] * aL[i];
+#pragma unroll
+              for (int d = 0; d < 2; ++d) {
+                const uint32_t hh = pk_bf16(k[2 * d], k[2 * d + 1]);
+                hw[e][d] = hh;
+                lw[e][d] = pk_bf16(k[2 * d] - bf16_low_f(hh), k[2 * d + 1] - bf16_high_f(hh));
+              }
+            } else {
+              hw[e][0] = hw[e][1] = lw[e][0] = lw[e][1] = 0u;
+            }
           }
-          D[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(t, bop, D[mb], 0, 0, 0);
+          V[b][p][0] = gt_u4{lw[0][0], lw[0][1], hw[0][0], hw[0][1]};
+          V[b][p][1] = gt_u4{hw[1][0], hw[1][1], lw[1][0], lw[1][1]};
         }
       }
-      // rows 16 kb + 4g + i are (o0, r) = (RC kb + oc, jl0 + i): scale by A_O0[o0]
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb) {
-        const float wv = buf[(mb * 16 + q) * RW + 2 * R + oc * NKB + kb];
+      for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(D[mb][i], wv, dacc[mb][i]);
+        for (int i = 0; i < 4; ++i) dacc[2 * h + b][i] = 0.f;
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb) {
+        f4 D[2];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) D[b] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int mu = 0; mu < NMF; ++mu) {
+          const b8 t = __builtin_bit_cast(b8, tab[(kb * NMF + mu) * kWave]);
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            b8 bop;
+            if (mu < NHL) { // HL: block p = mu / 2, first or second half
+              bop = __builtin_bit_cast(b8, V[b][mu / 2][mu & 1]);
+            } else { // HH: the middle of block p = mu - NHL
+              const int p = mu - NHL;
+              bop = __builtin_bit_cast(
+                  b8, (gt_u4{V[b][p][0][2], V[b][p][0][3], V[b][p][1][0], V[b][p][1][1]}));
+            }
+            D[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(t, bop, D[b], 0, 0, 0);
+          }
+        }
+        // rows 16 kb + 4g + i are (o0, r) = (RC kb + oc, jl0 + i): scale by A_O0[o0]
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          const float wv = buf[((2 * h + b) * 16 + q) * RW + 2 * R + oc * NKB + kb];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            dacc[2 * h + b][i] = __builtin_fmaf(D[b][i], wv, dacc[2 * h + b][i]);
+        }
       }
     }
     // the next tile's rows and the stream loads of the tile after it (issued
