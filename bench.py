@@ -73,7 +73,7 @@ _PATH_KERNEL = {"fp32": "k_gram_valu<{R}>", "fp32-wide": "k_gram_fp32<{R}>",
                 "bf16": "k_gram_bf16<{R}, 1>", "bf16r": "k_gram_bf16<{R}, 2>",
                 "bf16r-l2": "k_gram_bf16_o5<{R}>", "bf16r-wide": "k_gram_bf16w<{R}>",
                 "bf16r-ob": "k_gram_bf16_ob<{R}, {NS}, true>",
-                "bf16r-ob-l2": "k_gram_bf16_ob<{R}, {NS}, false>", "bf16r-kr": "k_gram_bf16_kr<{R}>"}
+                "bf16r-ob-l2": "k_gram_bf16_ob<{R}, {NS}, false>", "bf16r-kr": "k_gram_bf16_kr<{R}>", "bf16r-kr5": "k_gram_bf16_kr5<{R}>"}
 # the arithmetic each path's delta runs in
 _PATH_DTYPE = {"fp32": "fp32", "fp32-wide": "fp32", "bf16": "bf16", "bf16r": "bf16r",
                "bf16r-l2": "bf16r", "bf16r-wide": "bf16r", "bf16r-ob": "bf16r",
@@ -105,6 +105,8 @@ def measured_traffic(workload, kernel):
 
 def kernel_name(path, R, N=3):
     """The delta + Gram kernel of a gt_delta_path at kernel rank R."""
+    if path == "bf16r-kr" and N == 5:
+        path = "bf16r-kr5"
     return _PATH_KERNEL[path].format(R=R, NS=max(N - 2, 0))
 
 

@@ -468,7 +468,7 @@ std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R
   const size_t n16 = gb_k16_words(O), n32 = gb_k32_words(O);
   const size_t npair = bf16r_pair_table(R) ? gb_k32_pair_words(O) : 0;
   const size_t nob = bf16r_ob_table(R, N, O) ? gb_ob_words(R, O) : 0;
-  const size_t nkr = bf16r_kr_table(R, N) ? gb_kr_words(R) : 0;
+  const size_t nkr = bf16r_kr_table(R, N) ? gb_kr_words(R, N) : 0;
   std::vector<uint16_t> tab(n16 + 2 * n32 + npair + nob + nkr, 0);
   auto g_at = [&](int o, int jL, int r) -> float {
     return (o < O && jL < R && r < R) ? gt[((size_t)o * R + jL) * R + r] : 0.f;
@@ -526,12 +526,16 @@ std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R
   // entries e = o1 * R + jL the group takes in MFMA mu
   const size_t kr_at = gb_kr_offset(R, N, O);
   const int nhl = kr_nhl(R), nmf = kr_nhl(R) + kr_nhh(R);
-  for (int kb = 0; nkr && kb < kr_nkb(R); ++kb)
+  for (int kb = 0; nkr && kb < kr_nkb(R, N); ++kb)
     for (int mu = 0; mu < nmf; ++mu)
       for (int l = 0; l < kWave; ++l) {
-        const int q = l % 16, g = l / 16, rho = 16 * kb + q, o0 = rho / R, r = rho % R;
-        auto g_e = [&](int e) { // G[o0][o1][jL][r] of entry e, zero past R^2
-          return e < R * R ? gt[(((size_t)o0 * R + e / R) * R + e % R) * R + r] : 0.f;
+        const int q = l % 16, g = l / 16, rho = 16 * kb + q;
+        // output row: (o0, r) at order 4; (o01, r) with r padded to 8 at order 5
+        const int oh = N == 4 ? rho / R : rho / 8, r = N == 4 ? rho % R : rho % 8;
+        const bool row_ok = r < R && oh < (N == 4 ? R : R * R);
+        auto g_e = [&](int e) { // G[oh][o_last][jL][r] of entry e, zero past R^2
+          return (row_ok && e < R * R) ? gt[(((size_t)oh * R + e / R) * R + e % R) * R + r]
+                                       : 0.f;
         };
         uint16_t *w = tab.data() + kr_at + (((size_t)kb * nmf + mu) * kWave + l) * 8;
         if (mu < nhl) { // k_lo.G_hi + k_hi.G_hi over entries 16 mu + 4g + i

@@ -998,12 +998,12 @@ template <int R>
 __global__ void __launch_bounds__(64 * kTableWaves) __attribute__((amdgpu_waves_per_eu(3)))
 k_gram_bf16_kr(GramArgs a, int wpb) {
   static_assert(R == 4 || R == 8, "the Kronecker kernel covers order 4 at R = 4 and 8");
-  constexpr int NHL = kr_nhl(R), NHH = kr_nhh(R), NMF = NHL + NHH, NKB = kr_nkb(R);
+  constexpr int NHL = kr_nhl(R), NHH = kr_nhh(R), NMF = NHL + NHH, NKB = kr_nkb(R, 4);
   constexpr int NP = (NHL + 1) / 2; // 8-word operand blocks per block of 16 nonzeros
   constexpr int T = ChunkGram<R>::T;
   constexpr int RW = 3 * R; // floats per nonzero in the row area: A_L, A_O1, A_O0
   constexpr size_t kBuf = kr_lds_floats_per_wave<R>() - kWave;
-  constexpr size_t kTabFloats = gb_kr_words(R) / 2;
+  constexpr size_t kTabFloats = gb_kr_words(R, 4) / 2;
   extern __shared__ __align__(16) float dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   {
@@ -1176,6 +1176,190 @@ k_gram_bf16_kr(GramArgs a, int wpb) {
   gram.finish();
 }
 
+// ------------------------------------------------------------ BF16, order 5, Kronecker
+// The order-5 form of the two-mode refined BF16 delta, at R = 5 (layout in
+// gt_internal.hpp).  With outer slots o0, o1, o2 and L the last other mode:
+//   stage 1 (MFMA): V[m][(o01, r)] = sum_{o2, jL} G[o01][o2][jL][r] k_m[o2 R + jL],
+//                   k_m = A_O2[i_O2(m)] (x) A_L[i_L(m)], split per nonzero,
+//   stage 2 (VALU): delta[m][r] = sum_{o01} s_m[o01] V[m][(o01, r)],
+//                   s_m[o01] = A_O0[i_O0(m)][o0] * A_O1[i_O1(m)][o1],
+// R FMAs per output where the o-block kernel takes R^3.  The 25 entries of k
+// are dense in the two HL MFMAs (a lane's (o2, jL) pairs come from a small
+// per-lane offset table); output rows pad r to 8 so a lane's rows are
+// (o01, r) = (2 kb + g / 2, 4 (g % 2) + i).  Each lane stages its own
+// nonzero's A_L and A_O2 rows and its 25 s values (permuted: even o01, then
+// odd) in this wave's LDS.
+template <int R> constexpr int kr5_rw() { // floats per nonzero in the row area
+  return 2 * R + 2 * kr_nkb(R, 5);
+}
+template <int R> constexpr size_t kr5_lds_floats_per_wave() {
+  return (size_t)((kr5_rw<R>() * kWave > 16 * kDP) ? kr5_rw<R>() * kWave : 16 * kDP) + kWave;
+}
+
+template <int R>
+__global__ void __launch_bounds__(64 * kTableWaves) __attribute__((amdgpu_waves_per_eu(3)))
+k_gram_bf16_kr5(GramArgs a, int wpb) {
+  static_assert(R == 5, "the order-5 Kronecker kernel covers R = 5");
+  constexpr int NHL = kr_nhl(R), NHH = kr_nhh(R), NMF = NHL + NHH, NKB = kr_nkb(R, 5);
+  static_assert(NHL == 2 && NHH == 1, "one operand block of two HL and one HH MFMA");
+  constexpr int T = ChunkGram<R>::T;
+  constexpr int RW = kr5_rw<R>();   // A_L, A_O2, s (even o01, then odd)
+  constexpr int SO = 2 * R;         // offset of the s values
+  constexpr size_t kBuf = kr5_lds_floats_per_wave<R>() - kWave;
+  constexpr size_t kTabFloats = gb_kr_words(R, 5) / 2;
+  extern __shared__ __align__(16) float dyn[];
+  const int lane = threadIdx.x & (kWave - 1), w = wave_index();
+  {
+    const uint4 *src = reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(a.Gb) +
+                                                       gb_kr_offset(R, 5, R * R * R));
+    uint4 *dst = reinterpret_cast<uint4 *>(dyn);
+    for (int i = threadIdx.x; i < NKB * NMF * kWave; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+  }
+  const s8 *tab = reinterpret_cast<const s8 *>(dyn) + lane;
+  const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
+  if (c >= a.chunk_end) return;
+  float *buf = dyn + kTabFloats + (size_t)w * kr5_lds_floats_per_wave<R>();
+  float *tile = buf; // after stage 2; the row area before it
+  uint32_t *s_rowv = reinterpret_cast<uint32_t *>(buf + kBuf);
+  const uint64_t cb = c * a.chunk;
+  const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
+  const int q = lane & 15, g = lane >> 4;
+  // this lane group's entries e = 16 mu + 4g + i: LDS offsets of A_L[jL] and
+  // A_O2[o2] in a nonzero's row area (an entry past R^2 reads entry 0; its
+  // core words are zero)
+  uint32_t eoff[NHL][4];
+#pragma unroll
+  for (int mu = 0; mu < NHL; ++mu)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e0 = 16 * mu + 4 * g + i, e = e0 < R * R ? e0 : 0;
+      eoff[mu][i] = (uint32_t)(e % R) | ((uint32_t)(R + e / R) << 16);
+    }
+  const int sp = (g >> 1) * NKB; // this lane's s values: o01 = 2 kb + g / 2
+  ChunkGram<R> gram(a, c, cb, ce, lane);
+
+  uint32_t n_row, n_iL, n_io[3];
+  float n_x;
+  auto load_idx = [&](uint64_t tb) {
+    const uint64_t e = tb + lane;
+    const uint64_t ee = e < ce ? e : cb;
+    n_row = a.idx_row[ee];
+    n_iL = a.idx_L[ee];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) n_io[t] = a.idx_O[t][ee];
+    n_x = e < ce ? a.st.val[ee] : 0.f;
+  };
+  float rL[row_stride(R)], ro[3][row_stride(R)]; // this lane's nonzero's rows
+  auto gather = [&]() {
+    load_row<R>(a.A_L, n_iL, rL);
+#pragma unroll
+    for (int t = 0; t < 3; ++t) load_row<R>(a.A_O[t], n_io[t], ro[t]);
+  };
+  load_idx(cb);
+  uint32_t c_row = n_row;
+  float c_x = n_x;
+  gather();
+  load_idx(cb + kWave);
+
+  for (uint64_t tb = cb; tb < ce; tb += kWave) {
+    const bool valid = tb + lane < ce;
+    const int cnt = (ce - tb < (uint64_t)kWave) ? (int)(ce - tb) : kWave;
+    const uint32_t row = c_row;
+    const float x = c_x;
+    c_row = n_row;
+    c_x = n_x;
+    s_rowv[lane] = valid ? row : 0xFFFFFFFFu;
+    {
+      float *my = buf + lane * RW;
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        my[j] = rL[j];
+        my[R + j] = ro[2][j];
+      }
+#pragma unroll
+      for (int o = 0; o < R * R; ++o) my[SO + (o % 2) * NKB + o / 2] = ro[0][o / R] * ro[1][o % R];
+      if constexpr (R * R % 2 == 1) my[SO + NKB + NKB - 1] = 0.f; // the odd half's pad
+    }
+    wave_sync();
+    // [lo(k[entries of mu 0]), hi(...), hi(k[mu 1]), lo(...)] per block of 16
+    gt_u4 V[4][2];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const float *rm = buf + (mb * 16 + q) * RW;
+      uint32_t hw[2][2], lw[2][2];
+#pragma unroll
+      for (int mu = 0; mu < NHL; ++mu) {
+        float k[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) k[i] = rm[eoff[mu][i] >> 16] * rm[eoff[mu][i] & 0xFFFFu];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          const uint32_t hh = pk_bf16(k[2 * d], k[2 * d + 1]);
+          hw[mu][d] = hh;
+          lw[mu][d] = pk_bf16(k[2 * d] - bf16_low_f(hh), k[2 * d + 1] - bf16_high_f(hh));
+        }
+      }
+      V[mb][0] = gt_u4{lw[0][0], lw[0][1], hw[0][0], hw[0][1]};
+      V[mb][1] = gt_u4{hw[1][0], hw[1][1], lw[1][0], lw[1][1]};
+    }
+    float dacc[4][4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dacc[mb][i] = 0.f;
+#pragma unroll 1
+    for (int kb = 0; kb < NKB; ++kb) {
+      f4 D[4];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) D[mb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int mu = 0; mu < NMF; ++mu) {
+        const b8 t = __builtin_bit_cast(b8, tab[(kb * NMF + mu) * kWave]);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+          const gt_u4 v = mu < NHL ? V[mb][mu]
+                                   : gt_u4{V[mb][0][2], V[mb][0][3], V[mb][1][0], V[mb][1][1]};
+          D[mb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(t, __builtin_bit_cast(b8, v), D[mb], 0,
+                                                          0, 0);
+        }
+      }
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const float wv = buf[(mb * 16 + q) * RW + SO + sp + kb];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dacc[mb][i] = __builtin_fmaf(D[mb][i], wv, dacc[mb][i]);
+      }
+    }
+    // the next tile's rows and the stream loads of the tile after it
+    gather();
+    load_idx(tb + 2 * kWave);
+    // lane groups g and g + 2 hold the same columns for the other parity of o01
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) dacc[mb][i] = add_xor32(dacc[mb][i]);
+    wave_sync(); // every read of the row area before the tile overwrites it
+    // lane (q, 0) holds delta[16mb + q][0..3], lane (q, 1) delta[16mb + q][4];
+    // then x over column R and zeros up to the Gram's last column tile
+    if (g < 2) {
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (4 * g + i < R) tile[(4 * g + i) * kDP + mb * 16 + q] = dacc[mb][i];
+    }
+    tile[R * kDP + lane] = x;
+#pragma unroll
+    for (int r = R + 1; r < 4 * T; ++r) tile[r * kDP + lane] = 0.f;
+    wave_sync();
+    dump_tile<R>(a, tile, tb, cnt, lane);
+    gram.add_tile(tile, s_rowv, tb, cnt);
+    wave_sync();
+  }
+  gram.finish();
+}
+
 // ------------------------------------------------------------ split factor
 // Row i of the split copy holds, for each lane group g, the 16 bytes of BF16
 // words that group's K=32 slots take (bf16r_plan), so the gather of one
@@ -1258,13 +1442,32 @@ template <int R> int launch_kr(const GramArgs &a, hipStream_t st, bool any) {
   if constexpr (!bf16r_kr_table(R, 4)) {
     return -1;
   } else {
-    const size_t lds = gb_kr_words(R) * sizeof(uint16_t) +
+    const size_t lds = gb_kr_words(R, 4) * sizeof(uint16_t) +
                        (size_t)kTableWaves * kr_lds_floats_per_wave<R>() * sizeof(float);
     if (lds > lds_limit()) return -1;
     if (any) {
       const unsigned grid =
           (unsigned)((a.chunk_end - a.chunk_begin + kTableWaves - 1) / kTableWaves);
       hipLaunchKernelGGL((k_gram_bf16_kr<R>), dim3(grid), dim3(64 * kTableWaves), lds, st, a,
+                         kTableWaves);
+    }
+    return GT_DELTA_BF16R_KR;
+  }
+}
+
+// Launches k_gram_bf16_kr5 over the chunks of `a` (if any); -1 where it is
+// not built for R or the table and twelve waves do not fit in LDS.
+template <int R> int launch_kr5(const GramArgs &a, hipStream_t st, bool any) {
+  if constexpr (!bf16r_kr_table(R, 5)) {
+    return -1;
+  } else {
+    const size_t lds = gb_kr_words(R, 5) * sizeof(uint16_t) +
+                       (size_t)kTableWaves * kr5_lds_floats_per_wave<R>() * sizeof(float);
+    if (lds > lds_limit()) return -1;
+    if (any) {
+      const unsigned grid =
+          (unsigned)((a.chunk_end - a.chunk_begin + kTableWaves - 1) / kTableWaves);
+      hipLaunchKernelGGL((k_gram_bf16_kr5<R>), dim3(grid), dim3(64 * kTableWaves), lds, st, a,
                          kTableWaves);
     }
     return GT_DELTA_BF16R_KR;
@@ -1287,16 +1490,17 @@ template <int R, int KIND> int launch_impl(const GramArgs &a, int O, hipStream_t
       // orders 4-5: o-blocks up to kObMaxO outer indices, then (order 5,
       // R <= 8) pairs of outer indices
       if (a.As_L != nullptr && (a.slots == 2 || a.slots == 3) && O == cpow(R, a.slots)) {
-        // order 4 at R = 4, 8: two modes on the matrix cores; the other
-        // source (a.alt_source) is the o-block kernel with its table in LDS
-        if (a.slots == 2 && bf16r_kr_table(R, 4)) {
+        // order 4 at R = 4, 8 and order 5 at R = 5: two modes on the matrix
+        // cores; the other source (a.alt_source) is the o-block kernel with
+        // its table in LDS
+        if (bf16r_kr_table(R, a.slots + 2)) {
           if (!a.alt_source) {
-            const int p = launch_kr<R>(a, st, any);
+            const int p = a.slots == 2 ? launch_kr<R>(a, st, any) : launch_kr5<R>(a, st, any);
             if (p >= 0) return p;
           } else {
             GramArgs b = a;
             b.alt_source = 0;
-            const int p = launch_ob<R, 2>(b, st, any);
+            const int p = a.slots == 2 ? launch_ob<R, 2>(b, st, any) : launch_ob<R, 3>(b, st, any);
             if (p >= 0) return p;
           }
         }

@@ -260,17 +260,24 @@ __host__ __device__ constexpr size_t gb_ob_words(int R, int O) {
 // core words G_lo).  Even/odd orders let one 8-word register block per pair
 // of mu hold both HL operands and the HH operand between them.  R divides
 // 16, so a lane's jL = 4g % R + i and o1 = (16 mu + 4g) / R separate.
+// Order 5 at R = 5 takes the same table form with (o2, jL) contracted:
+// e = o2 * R + jL (dense, R^2 = 25 entries; a lane group's (o2, jL) do not
+// separate, so each lane keeps their LDS offsets), and output rows (o01, r)
+// with o01 = o0 * R + o1 and r padded to 8 (rho = 8 o01 + r, rows with
+// r >= R zero), so a lane's o01 = 2 kb + g / 2 and r = 4 (g % 2) + i.
 __host__ __device__ constexpr bool bf16r_kr_table(int R, int N) {
-  return N == 4 && (R == 4 || R == 8);
+  return (N == 4 && (R == 4 || R == 8)) || (N == 5 && R == 5);
 }
-__host__ __device__ constexpr int kr_nhl(int R) { return R * R / 16; }
+__host__ __device__ constexpr int kr_nhl(int R) { return (R * R + 15) / 16; }
 __host__ __device__ constexpr int kr_nhh(int R) { return (R * R + 31) / 32; }
-__host__ __device__ constexpr int kr_nkb(int R) { return R * R / 16; }
+__host__ __device__ constexpr int kr_nkb(int R, int N) {
+  return N == 4 ? R * R / 16 : (8 * R * R + 15) / 16;
+}
 __host__ __device__ constexpr size_t gb_kr_offset(int R, int N, int O) {
   return gb_ob_offset(R, O) + (bf16r_ob_table(R, N, O) ? gb_ob_words(R, O) : 0);
 }
-__host__ __device__ constexpr size_t gb_kr_words(int R) {
-  return (size_t)kr_nkb(R) * (kr_nhl(R) + kr_nhh(R)) * kWave * 8;
+__host__ __device__ constexpr size_t gb_kr_words(int R, int N) {
+  return (size_t)kr_nkb(R, N) * (kr_nhl(R) + kr_nhh(R)) * kWave * 8;
 }
 // Returns the gt_delta_path it took.
 int launch_gram_mfma(int R, int precision, const GramArgs &a, int O, hipStream_t s);

@@ -63,9 +63,10 @@ typedef enum gt_delta_path {
                               per 16 outer indices, the o-block table copied to LDS per
                               workgroup of 12 waves */
   GT_DELTA_BF16R_OB_L2 = 8, /* the same, o-block table from L2 */
-  GT_DELTA_BF16R_KR = 9     /* refined BF16 MFMA delta of order 4 at R = 4 and 8 with two modes
-                               contracted on the matrix cores (the Kronecker operand A_O1 x A_L
-                               split per nonzero), table in LDS per workgroup of 12 waves */
+  GT_DELTA_BF16R_KR = 9     /* refined BF16 MFMA delta of order 4 at R = 4 and 8 and of order 5
+                               at R = 5 with two modes contracted on the matrix cores (the
+                               Kronecker operand of the last outer factor and A_L, split per
+                               nonzero), table in LDS per workgroup of 12 waves */
 } gt_delta_path;
 
 typedef struct gt_opts {

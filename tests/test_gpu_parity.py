@@ -279,8 +279,9 @@ def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
     R = 10 the refinement takes a second MFMA per tile; orders 4 and 5 run
     the refined delta on the o-block kernel, 16 outer indices per MFMA, at
     R = 5, 7 and 9 of order 4 and R = 4, 5 of order 5 with partial blocks
-    or none, and at order 4 with R = 4 and 8 the Kronecker kernel, two modes
-    per MFMA; plain BF16 there takes the order-3 kernel's loop over outer
+    or none, and at order 4 with R = 4 and 8 and order 5 with R = 5 the
+    Kronecker kernel, two modes per MFMA; plain BF16 there takes the order-3
+    kernel's loop over outer
     indices) and the FP32 VALU delta (two tiles per pass) feeding the MFMA
     Gram, against the FP64 oracle's B and C."""
     if R > 15 and prec == "bf16":
@@ -606,14 +607,15 @@ def test_autotune_keeps_parity(prec, N, R, part):
     (4, 8, [1, 1, 2, 1], ("bf16r-kr", "bf16r-ob")),
     (4, 4, [2, 1, 1, 1], ("bf16r-kr", "bf16r-ob")),
     (4, 10, [1, 1, 1, 2], ("bf16r-ob", "bf16r-ob-l2")),
-    (5, 5, [1, 1, 1, 2, 1], ("bf16r-ob", "bf16r-ob-l2")),
+    (5, 5, [1, 1, 1, 2, 1], ("bf16r-kr", "bf16r-ob")),
     (5, 2, [1, 1, 1, 1, 2], ("bf16r-ob", "bf16r-ob-l2")),
     (5, 6, [1, 1, 2, 1, 1], ("bf16r-l2", "bf16r-l2"))])
 def test_core_source_variants(N, R, part, paths):
     """Both sources of the refined BF16 core operand (core_source 0 and 1):
     the o-block kernel (orders 4-5 with R^(N-2) <= 128, the o-block table in
     LDS or from L2, with partial last blocks at R = 5 and 10; at order 4 with
-    R = 4 and 8 the Kronecker kernel and the o-block kernel) and, above
+    R = 4 and 8 and order 5 with R = 5 the Kronecker kernel and the o-block
+    kernel) and, above
     that, the pair kernel (its pair table, 83 KiB at order 5 and R = 6, does
     not go in LDS beside twelve waves: L2 either way).  Each one's fused
     deltas against the FP64 oracle's (delta.cuh:62-90), and its B and C
