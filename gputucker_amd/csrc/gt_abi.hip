@@ -573,10 +573,13 @@ std::vector<uint16_t> bf16_core_table_wide(const std::vector<float> &gt, int R) 
   return tab;
 }
 
-// Upper bound of the default chunk size, in 64-nonzero tiles.  Past about
-// 16384 chunks the partial tables and the fix-up get cheaper with longer
-// chunks, up to 32 tiles; at 64 the order-3 kernel slows (too few waves).
-constexpr uint64_t kMaxChunkTiles = 32;
+// Bounds of the default chunk size, in 64-nonzero tiles.  Past about 16384
+// chunks the partial tables and the fix-up get cheaper with longer chunks,
+// up to 32 tiles; at 64 the order-3 kernel slows (too few waves).  At least
+// two: the FP32 delta forms two tiles per pass (order 3 and R^N <= 4096),
+// and on a 1e6-nonzero tensor one-tile chunks cost it 30 % (the refined
+// BF16 delta 4 %).
+constexpr uint64_t kMinChunkTiles = 2, kMaxChunkTiles = 32;
 
 void ensure_work(gt_ctx *c) {
   uint64_t max_rows = 0;
@@ -587,7 +590,7 @@ void ensure_work(gt_ctx *c) {
   if (tiles == 0) { // enough chunks to give every SIMD several waves
     const uint64_t target_waves = 16384;
     tiles = c->nnz / (kTile * target_waves);
-    tiles = std::min<uint64_t>(std::max<uint64_t>(tiles, 1), kMaxChunkTiles);
+    tiles = std::min<uint64_t>(std::max<uint64_t>(tiles, kMinChunkTiles), kMaxChunkTiles);
   }
   c->chunk = tiles * kTile;
   c->nchunks = (c->nnz + c->chunk - 1) / c->chunk;
@@ -692,7 +695,9 @@ uint64_t incore_partition_bytes_per_nnz(int N) {
 uint64_t fixed_estimate(const gt_ctx *c, uint64_t nnz, const uint32_t *dims) {
   const int N = c->N, R = c->R, W = gram_width(R);
   uint64_t tiles = (uint64_t)c->o.chunk_tiles;
-  if (tiles == 0) tiles = std::min<uint64_t>(std::max<uint64_t>(nnz / (kTile * 16384), 1), kMaxChunkTiles);
+  if (tiles == 0)
+    tiles = std::min<uint64_t>(std::max<uint64_t>(nnz / (kTile * 16384), kMinChunkTiles),
+                               kMaxChunkTiles);
   const uint64_t nchunks = (nnz + tiles * kTile - 1) / (tiles * kTile) + 1;
   const uint64_t split = c->o.precision == GT_PREC_FP32 ? 0
                          : (R <= 15 ? kSplitWords : kSplitWordsWide);

@@ -82,7 +82,7 @@ typedef struct gt_opts {
                                        When the resident layout exceeds it, the per-mode
                                        streams are staged from pinned host memory */
   int chunk_tiles;                  /* 64-nonzero tiles per Gram chunk; 0 = from the nonzero
-                                       count: about 16384 chunks, 1 to 32 tiles each */
+                                       count: about 16384 chunks, 2 to 32 tiles each */
   int profile;                      /* 1: time the Gram kernel with HIP events */
   int ref_fit;                      /* 0: the fit from FP64 sums.  1: the fit in the reference's
                                        value_t arithmetic: the norm as a serial float sum of x^2
