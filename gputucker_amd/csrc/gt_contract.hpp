@@ -192,51 +192,75 @@ contract_slots(int slots, const float *__restrict__ A_L, uint32_t iL,
   }
 }
 
-// The per-row Gram accumulation of a chunk with one lane per Gram entry (the
-// wide ranks, R > 15): lane q owns entries q, q + 64, ... of the packed row
-// (upper triangle of B, then C = sum x delta) and walks each tile's
-// nonzeros in order with FP64 products of the FP32 deltas, flushing on every
-// row change: complete rows to the Gram table, the chunk's first and last
-// runs, when they cross its edges, to the partial tables of the fix-up.  The
-// tile is kWave rows of pitch RP in LDS, deltas in columns < R and x in
-// column R; `srow` holds the tiles' row ids.
-// nonzeros per step of LaneGram's whole-tile loop
-constexpr int kLaneGramUnroll = 16;
+// The per-row Gram accumulation of a chunk for the wide ranks (R > 15), one
+// block of Gram entries per lane.  The entries are those of the (R+1)-column
+// matrix [delta | x] against itself: (p, c) with p <= c < R is B(p, c), (p, R)
+// is C(p) = sum x delta_p.  Lane l owns rows p0 = 2i and p0 + 1 against the K
+// columns c0 .. c0 + K - 1 of one block; the blocks of row pair i start at
+// column 2i and tile the row pair's columns up to R, and K is the least width
+// whose blocks fit in one wave (K = 5 at R = 32: 64 blocks).  Per nonzero a
+// lane reads K + 2 values and adds 2K FP64 products of the FP32 deltas (K + 2
+// reads and conversions for 2K entries, where one entry per lane takes two of
+// each); entries below the diagonal or past column R are formed and dropped.
+// Each entry is summed over the nonzeros in stream order, so the sums are
+// those of one accumulator per entry.  The wave walks each tile's nonzeros in
+// order and flushes on every row change: complete rows to the Gram table,
+// the chunk's first and last runs, when they cross its edges, to the partial
+// tables of the fix-up.  The tile is kWave rows of pitch RP >= R + 1 in LDS,
+// deltas in columns < R and x in column R; `srow` holds the tiles' row ids.
+template <int R> struct GramBlocks {
+  static constexpr int blocks(int K) {
+    int n = 0;
+    for (int i = 0; 2 * i < R; ++i) n += (R + 1 - 2 * i + K - 1) / K;
+    return n;
+  }
+  static constexpr int width() {
+    int K = 1;
+    while (blocks(K) > kWave) ++K;
+    return K;
+  }
+  static constexpr int K = width();
+  static constexpr int NB = blocks(K);
+};
+// nonzeros per step of BlockGram's whole-tile loop
+constexpr int kBlockGramUnroll = 4;
 
-template <int R, int RP> struct LaneGram {
+template <int R, int RP> struct BlockGram {
+  static_assert(RP >= R + 1, "the tile holds columns 0..R");
   static constexpr int W = gram_width(R);
-  static constexpr int NE = gram_entries(R);
-  static constexpr int E = (NE + kWave - 1) / kWave; // entries per lane
+  static constexpr int K = GramBlocks<R>::K;
   const GramArgs &a;
   uint64_t c, cb, ce;
   int lane;
-  uint32_t off[E]; // the entry's two tile columns, p | q << 16 (p = q = 0 past NE)
-  double acc[E];
+  int p0, c0;      // the lane's block: rows p0, p0 + 1, columns c0 .. c0 + K - 1 (p0 = -1: none)
+  int rd[2], cd[K]; // the tile columns read for them, clamped to column R
+  double acc[2][K];
   uint32_t cur, last_row;
   uint64_t run_begin;
   bool head_open, tail_open;
 
-  __device__ LaneGram(const GramArgs &args, uint64_t chunk, uint64_t b, uint64_t e, int l)
+  __device__ BlockGram(const GramArgs &args, uint64_t chunk, uint64_t b, uint64_t e, int l)
       : a(args), c(chunk), cb(b), ce(e), lane(l) {
-    // entry ownership: q < R(R+1)/2 -> B(p, p2), p <= p2; else C(p) = delta_p * x
-#pragma unroll
-    for (int qi = 0; qi < E; ++qi) {
-      const int q = lane + qi * kWave;
-      int p = 0, p2 = 0;
-      if (q < R * (R + 1) / 2) {
-        int rem = q;
-        while (rem >= R - p) {
-          rem -= R - p;
-          ++p;
-        }
-        p2 = p + rem;
-      } else if (q < NE) {
-        p = q - R * (R + 1) / 2;
-        p2 = R;
+    p0 = -1;
+    c0 = 0;
+    int n = 0;
+    for (int i = 0; 2 * i < R; ++i) {
+      const int nb = (R + 1 - 2 * i + K - 1) / K;
+      if (lane >= n && lane < n + nb) {
+        p0 = 2 * i;
+        c0 = 2 * i + (lane - n) * K;
       }
-      off[qi] = (uint32_t)p | ((uint32_t)p2 << 16);
-      acc[qi] = 0.0;
+      n += nb;
     }
+    const int pr = p0 < 0 ? 0 : p0;
+    rd[0] = pr;
+    rd[1] = pr + 1 <= R ? pr + 1 : R;
+#pragma unroll
+    for (int k = 0; k < K; ++k) cd[k] = c0 + k <= R ? c0 + k : R;
+#pragma unroll
+    for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[rr][k] = 0.0;
     cur = __builtin_amdgcn_readfirstlane(a.idx_row[cb]);
     run_begin = cb;
     // Runs other than the chunk's first and last are whole rows.
@@ -255,35 +279,61 @@ template <int R, int RP> struct LaneGram {
     else
       dst = a.bpart1 + (size_t)c * W;
 #pragma unroll
-    for (int qi = 0; qi < E; ++qi)
-      if (lane + qi * kWave < NE) dst[lane + qi * kWave] = acc[qi];
+    for (int rr = 0; rr < 2; ++rr) {
+      const int p = p0 + rr;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int cc = c0 + k;
+        if (p0 >= 0 && p < R && cc >= p && cc <= R) {
+          const int at = cc < R ? p * R - p * (p - 1) / 2 + (cc - p) : R * (R + 1) / 2 + p;
+          dst[at] = acc[rr][k];
+        }
+        acc[rr][k] = 0.0;
+      }
+    }
+  }
+
+  __device__ void add_row(const float *tr) {
+    const double r0 = (double)tr[rd[0]], r1 = (double)tr[rd[1]];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double ck = (double)tr[cd[k]];
+      acc[0][k] += r0 * ck;
+      acc[1][k] += r1 * ck;
+    }
   }
 
   // Adds tile [tb, tb + cnt); tile and srow must be visible to the whole wave.
   __device__ void add_tile(const float *tile, const uint32_t *srow, uint64_t tb, int cnt) {
     // A whole tile that continues the current run (the common case inside
-    // long rows) needs no row checks: each entry's two columns are read
-    // through per-lane pointers with the nonzero's offset as an immediate,
-    // so the loop spends no arithmetic on addresses.  Same products, same
-    // order of the sums as the general loop below.
+    // long rows) needs no row checks: the lane's columns are read through
+    // per-lane pointers with the nonzero's offset as an immediate, so the
+    // loop spends no arithmetic on addresses.  Same products, same order of
+    // the sums as the general loop below.
     if (cnt == kWave && __ballot(srow[lane] != cur) == 0) {
-      const float *bp[E], *bq[E];
+      // LDS pointers by type: through generic ones the compiler may read
+      // some columns with flat loads, which wait on both memory counters
+      typedef __attribute__((address_space(3))) const float lds_cfloat;
+      lds_cfloat *bp[2], *bq[K];
 #pragma unroll
-      for (int qi = 0; qi < E; ++qi) {
-        bp[qi] = tile + (off[qi] & 0xFFFFu);
-        bq[qi] = tile + (off[qi] >> 16);
-      }
-      for (int j0 = 0; j0 < kWave; j0 += kLaneGramUnroll) {
+      for (int rr = 0; rr < 2; ++rr) bp[rr] = (lds_cfloat *)(tile + rd[rr]);
 #pragma unroll
-        for (int j = 0; j < kLaneGramUnroll; ++j)
+      for (int k = 0; k < K; ++k) bq[k] = (lds_cfloat *)(tile + cd[k]);
+      for (int j0 = 0; j0 < kWave; j0 += kBlockGramUnroll) {
 #pragma unroll
-          for (int qi = 0; qi < E; ++qi)
-            acc[qi] += (double)bp[qi][j * RP] * (double)bq[qi][j * RP];
+        for (int j = 0; j < kBlockGramUnroll; ++j) {
+          const double r0 = (double)bp[0][j * RP], r1 = (double)bp[1][j * RP];
 #pragma unroll
-        for (int qi = 0; qi < E; ++qi) {
-          bp[qi] += kLaneGramUnroll * RP;
-          bq[qi] += kLaneGramUnroll * RP;
+          for (int k = 0; k < K; ++k) {
+            const double ck = (double)bq[k][j * RP];
+            acc[0][k] += r0 * ck;
+            acc[1][k] += r1 * ck;
+          }
         }
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr) bp[rr] += kBlockGramUnroll * RP;
+#pragma unroll
+        for (int k = 0; k < K; ++k) bq[k] += kBlockGramUnroll * RP;
       }
       return;
     }
@@ -293,13 +343,8 @@ template <int R, int RP> struct LaneGram {
         flush(tb + j);
         cur = row;
         run_begin = tb + j;
-#pragma unroll
-        for (int qi = 0; qi < E; ++qi) acc[qi] = 0.0;
       }
-      const float *tr = tile + j * RP;
-#pragma unroll
-      for (int qi = 0; qi < E; ++qi)
-        acc[qi] += (double)tr[off[qi] & 0xFFFFu] * (double)tr[off[qi] >> 16];
+      add_row(tile + j * RP);
     }
   }
 

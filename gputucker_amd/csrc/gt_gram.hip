@@ -9,10 +9,10 @@
 // B_i += delta delta^T, C_i += x delta on the host in double.  Here both
 // happen in one kernel: a wave owns a chunk of the stream (64 * tiles
 // nonzeros, rows ascending), each lane computes one nonzero's delta per tile,
-// the tile goes to LDS, and the lanes then switch roles -- lane q owns Gram
-// entry q (upper triangle of B, then C) and walks the tile's nonzeros in
-// order, flushing its FP64 accumulator whenever the row changes.  delta never
-// reaches HBM.  Products of the FP32 deltas are formed exactly in FP64 and
+// the tile goes to LDS, and the lanes then switch roles -- each lane owns a
+// block of Gram entries (two rows of the upper triangle of B, or of C,
+// against a few columns) and walks the tile's nonzeros in order, flushing its
+// FP64 accumulators whenever the row changes.  delta never reaches HBM.  Products of the FP32 deltas are formed exactly in FP64 and
 // summed in FP64 (the reference rounds each product to value_t first).
 #include <atomic>
 
@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) k_gram_fp32(GramArgs a, int wpb) {
   const uint64_t cb = c * a.chunk;
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
   const uint32_t *__restrict__ rows = a.idx_row;
-  LaneGram<R, RP> gram(a, c, cb, ce, lane);
+  BlockGram<R, RP> gram(a, c, cb, ce, lane);
 
   for (uint64_t tb = cb; tb < ce; tb += kWave) {
     const uint64_t e = tb + lane;

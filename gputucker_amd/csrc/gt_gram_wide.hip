@@ -19,8 +19,8 @@
 // ahead.  A tile's stage-1 and stage-2 work (768 MFMAs and 1024 FMAs per lane
 // at R = 32) is long beside its gathers, so the gathers are not prefetched
 // across tiles: the other waves of the SIMD run while one waits.  The tile
-// of deltas then goes through LDS to the lane-per-entry FP64 Gram
-// (LaneGram), which reuses the LDS of the s values.
+// of deltas then goes through LDS to the FP64 Gram with a block of entries
+// per lane (BlockGram), which reuses the LDS of the s values.
 #include "gt_bf16.hpp"
 #include "gt_contract.hpp"
 
@@ -57,7 +57,7 @@ k_gram_bf16w(GramArgs a, int wpb) {
   const int q = lane & 15, g = lane >> 4;
   // core operand words, [o][mt][lane][hi, lo] (16 bytes each)
   const u4 *tab = reinterpret_cast<const u4 *>(a.Gb);
-  LaneGram<R, RP> gram(a, c, cb, ce, lane);
+  BlockGram<R, RP> gram(a, c, cb, ce, lane);
 
   for (uint64_t tb = cb; tb < ce; tb += kWave) {
     const uint64_t e = tb + lane;

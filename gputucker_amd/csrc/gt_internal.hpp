@@ -287,7 +287,7 @@ constexpr int kSplitWords = 16;
 void launch_split_factor(int R, const float *A, uint64_t rows, uint32_t *As, hipStream_t s);
 
 // The refined BF16 MFMA delta of the wide ranks (order 3, 16 <= R <= 32,
-// built for R = 16, 20, 24, 28, 30, 32) with the lane-per-entry FP64 Gram.
+// built for R = 16, 20, 24, 28, 30, 32) with the block-per-lane FP64 Gram.
 // It gathers a wide split copy of A_L (kSplitWordsWide words per row: per
 // lane group g, hi then lo of columns 8g..8g+7) and reads its core operand,
 // [o][mt][lane][hi 8, lo 8] BF16 words, from global memory.
