@@ -57,7 +57,7 @@ CONFIGS = {
     "c5": dict(name="5-order 5e4^4 x 200, 2e8 nnz uniform, R=5, 16 GB device budget",
                dims=[50_000] * 4 + [200], nnz=200_000_000, modes=None, values="uniform", rank=5,
                part=None, iters=2, lam=None, budget=16_000_000_000),
-    # R = 32 (the FP32 delta and the lane-per-entry Gram) with power-law rows
+    # R = 32 (the wide refined BF16 kernel, the block-per-lane FP64 Gram) with power-law rows
     # in mode 0; Zipf(1.5) in every mode would need about 2e10 draws for 3e8
     # distinct coordinates
     "c4": dict(name="3-order 5e5 x 5e5 x 5e4, 3e8 nnz Zipf(1.5) in mode 0, R=32",

@@ -52,13 +52,13 @@ typedef enum gt_precision {
 typedef enum gt_delta_path {
   GT_DELTA_NONE = 0,      /* no mode update has run */
   GT_DELTA_FP32 = 1,      /* FP32 VALU delta, FP64 MFMA Gram (R <= 15) */
-  GT_DELTA_FP32_WIDE = 2, /* FP32 VALU delta, lane-per-entry FP64 Gram (R > 15) */
+  GT_DELTA_FP32_WIDE = 2, /* FP32 VALU delta, block-per-lane FP64 Gram (R > 15) */
   GT_DELTA_BF16 = 3,      /* BF16 MFMA delta, core table in LDS */
   GT_DELTA_BF16R = 4,     /* refined BF16 MFMA delta, core table in LDS */
   GT_DELTA_BF16R_L2 = 5,  /* refined BF16 MFMA delta of order 5 at R = 6..8, one MFMA per two
                              outer indices, core operand from L2 */
   GT_DELTA_BF16R_WIDE = 6, /* refined BF16 MFMA delta of order 3 at 16 <= R <= 32, core operand
-                             from L2, lane-per-entry FP64 Gram */
+                             from L2, block-per-lane FP64 Gram */
   GT_DELTA_BF16R_OB = 7,   /* refined BF16 MFMA delta at orders 4-5 with R^(N-2) <= 128, one MFMA
                               per 16 outer indices, the o-block table copied to LDS per
                               workgroup of 12 waves */
