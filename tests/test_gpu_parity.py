@@ -300,6 +300,35 @@ def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
         assert (np.abs(Cg - Cv) / scc).max() < tol_gram
 
 
+@pytest.mark.parametrize("nnz", [1, 5, 63, 64, 65, 129])
+@pytest.mark.parametrize("N,R,prec", [(3, 10, "fp32"), (3, 10, "bf16r"), (4, 8, "bf16r"),
+                                      (5, 5, "bf16r"), (3, 32, "bf16r"), (3, 32, "fp32")])
+def test_tiny_tensors(nnz, N, R, prec):
+    """Tensors of one nonzero to a few partial tiles (a lone nonzero, one
+    partial tile, exactly one tile, one tile and one nonzero, two tiles and
+    one) through each kernel family: B and C against the FP64 oracle, then
+    a whole ALS iteration's fit within the precision's bar.  Every row here
+    has one or two nonzeros, so B = lambda I + a rank-one or rank-two term."""
+    print(f"tiny: N={N} R={R} nnz={nnz} {prec}", flush=True)
+    dims = [70, 60, 50, 40, 30][:N]
+    e, t, A, G = _setup(nnz * 7 + N + R, dims, nnz, R, [1] * N, precision=prec, lam=1e-3,
+                        max_iters=1, tol=0.0)
+    tol = 1e-5 if prec == "fp32" else 2e-4
+    for m in range(N):
+        d = oracle.delta(t, A, G, m)
+        B, Cv = oracle.bc(t, d, m, 1e-3)
+        Bg, Cg = e.get_gram(m, t.rows(m))
+        sc = np.abs(B).max(axis=(1, 2), keepdims=True)
+        assert (np.abs(Bg - B) / sc).max() < tol, (m, (np.abs(Bg - B) / sc).max())
+        scc = np.abs(Cv).max(axis=1, keepdims=True) + 1e-30
+        assert (np.abs(Cg - Cv) / scc).max() < tol, m
+    fits = e.run()
+    f64, _ = oracle.als(t, A, G, 1e-3, 1, 0.0, "f64")
+    # (a lone small value is shrunk to zero by lambda: both fits are then 0)
+    bar = 1e-5 if prec == "fp32" else 1e-4 * max(abs(f64[0]), 1e-2)
+    assert np.abs(fits - f64).max() <= bar, (fits, f64)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16r"])
 @pytest.mark.parametrize("N,R,part", [(4, 15, [1, 2, 1, 1]), (5, 12, [1, 1, 2, 1, 1])])
 def test_gram_parity_high_order_wide_rank(prec, N, R, part):
