@@ -22,6 +22,8 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
+import resource
 import statistics
 import sys
 import time
@@ -206,7 +208,8 @@ def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup
         r = run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup,
                        chunk_tiles, budget, False, autotune)
     e = gt.Engine(N, R, partition=part, lam=lam, precision=precision, chunk_tiles=chunk_tiles,
-                  profile=not graph, device_budget=budget, graph=graph, autotune=autotune)
+                  profile=not graph, device_budget=budget, graph=graph, autotune=autotune,
+                  verbose=1 if budget else 0)
     t0 = time.perf_counter()
     e.set_coo(idx, vals, dims)
     e.partition()
@@ -250,7 +253,8 @@ def run_engine(gt, idx, vals, dims, A, G, part, R, lam, precision, steps, warmup
                 partition_ms=tm["ms_partition"], part=[int(p) for p in info["part"]],
                 tuned_chunk_tiles=tm["tuned_chunk_tiles"], graph=graph,
                 partition_path=info["partition_path"],
-                stream_bytes_per_nnz=info["stream_bytes_per_nnz"])
+                stream_bytes_per_nnz=info["stream_bytes_per_nnz"],
+                host_pageable_bytes=info["host_pageable_bytes"])
 
 
 def main(argv=None):
@@ -267,6 +271,8 @@ def main(argv=None):
                     help="device budget in bytes; below the resident size the streams are "
                          "staged from pinned host memory every mode update (PCIe-inclusive)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--nnz", type=float, default=None,
+                    help="nonzeros of the workload (default: the config's)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="launch each iteration's kernels eagerly instead of replaying the "
                          "captured HIP graph")
@@ -278,7 +284,10 @@ def main(argv=None):
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is "
                                    "designed but not built (run with --gpus 1)"}))
         return 2
-    cfg = CONFIGS[a.config]
+    cfg = dict(CONFIGS[a.config])
+    if a.nnz is not None:
+        cfg["nnz"] = int(a.nnz)
+        cfg["name"] = re.sub(r"[0-9.e]+ nnz", f"{cfg['nnz']:.3g} nnz", cfg["name"])
     N, R = len(cfg["dims"]), cfg["rank"]
     if R > 15 and a.precision == "bf16":
         # plain BF16 is built for R <= 15; the wide ranks have the refined delta
@@ -299,7 +308,9 @@ def main(argv=None):
 
     t0 = time.perf_counter()
     idx, vals, dims = synth.generate(cfg["dims"], cfg["nnz"], cfg["modes"], cfg["values"],
-                                     seed=a.seed, device="cuda")
+                                     seed=a.seed, device="cuda",
+                                     log=lambda m: print(f"[bench] generating: {m}",
+                                                         file=sys.stderr, flush=True))
     max_deg = [int(np.bincount(idx[k]).max()) for k in range(N)]
     print(f"[bench] generated {cfg['name']}: {idx.shape[1]} nnz in "
           f"{time.perf_counter() - t0:.1f}s, largest row degree per mode {max_deg}",
@@ -317,7 +328,10 @@ def main(argv=None):
               f"{r['launches']} (gram phase {r['gram_phase_ms']:.3f} ms, solve "
               f"{r['solve_ms']:.3f} ms), out of core {r['out_of_core']} (segment "
               f"{r['segment_nnz']}), fit {r['fit']:.7f}, setup {r['setup_s']:.1f}s (partition "
-              f"{r['partition_ms']:.0f} ms, P {r['part']})", file=sys.stderr, flush=True)
+              f"{r['partition_ms']:.0f} ms, P {r['part']}), pageable host bytes "
+              f"{r['host_pageable_bytes']}, peak host RSS "
+              f"{resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 2**20:.1f} GiB",
+              file=sys.stderr, flush=True)
     head = runs[a.precision]
     paths = sorted(set(head["delta_path"]))
     dtype = "+".join(sorted({_PATH_DTYPE[p] for p in paths}))
@@ -370,7 +384,8 @@ def main(argv=None):
     if head["out_of_core"]:  # the host-link rate of the staged streams (never `value`)
         out["pcie"] = {"host_bytes_per_nnz": head["stream_bytes_per_nnz"],
                        "h2d_gbs": head["stream_bytes_per_nnz"] * idx.shape[1] / head["t_mode"] / 1e9,
-                       "partition_s": head["partition_ms"] / 1e3}
+                       "partition_s": head["partition_ms"] / 1e3,
+                       "host_pageable_bytes": head["host_pageable_bytes"]}
     if a.compare and a.compare in runs:
         base = runs[a.compare]
         out["compare"] = {"precision": a.compare, "value": base["value"],

@@ -45,6 +45,7 @@ class GtOpts(C.Structure):
         ("partition", C.c_uint32 * MAX_ORDER), ("device_budget", C.c_uint64),
         ("chunk_tiles", C.c_int), ("profile", C.c_int), ("ref_fit", C.c_int),
         ("autotune", C.c_int), ("graph", C.c_int), ("core_source", C.c_int),
+        ("host_pinned_limit", C.c_uint64), ("verbose", C.c_int),
     ]
 
 
@@ -58,6 +59,7 @@ class GtPartitionInfo(C.Structure):
         ("out_of_core", C.c_int), ("segment_nnz", C.c_uint64),
         ("stream_bytes_per_nnz", C.c_uint32), ("partition_path", C.c_int),
         ("partition_pieces", C.c_uint64),
+        ("host_pageable_bytes", C.c_uint64),
     ]
 
 
@@ -178,7 +180,7 @@ class Engine:
 
     def __init__(self, order, rank, device=0, lam=None, max_iters=3, tol=None, precision="fp32",
                  partition=None, device_budget=0, chunk_tiles=0, profile=False, ref_fit=False,
-                 graph=False, autotune=False, core_source=0):
+                 graph=False, autotune=False, core_source=0, host_pinned_limit=0, verbose=0):
         L = lib()
         o = GtOpts()
         L.gt_default_opts(C.byref(o))
@@ -199,6 +201,8 @@ class Engine:
         o.graph = 1 if graph else 0
         o.autotune = 1 if autotune else 0
         o.core_source = int(core_source)
+        o.host_pinned_limit = int(host_pinned_limit)
+        o.verbose = int(verbose)
         self.opts = o
         self.order, self.rank = int(order), int(rank)
         self._h = C.c_void_p()
@@ -260,7 +264,8 @@ class Engine:
                     type=i.partition_type, streams=i.stream_count, avail_nnz=i.avail_nnz_per_task,
                     out_of_core=i.out_of_core, segment_nnz=i.segment_nnz,
                     stream_bytes_per_nnz=i.stream_bytes_per_nnz, partition_path=i.partition_path,
-                    partition_pieces=i.partition_pieces)
+                    partition_pieces=i.partition_pieces,
+                    host_pageable_bytes=i.host_pageable_bytes)
 
     def get_block(self, b, n):
         idx = np.zeros((self.order, max(n, 1)), np.uint32)

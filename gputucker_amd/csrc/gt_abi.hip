@@ -22,11 +22,13 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <unordered_map>
 #include <vector>
 
 #include "gt_internal.hpp"
@@ -206,14 +208,82 @@ struct gt_ctx {
   uint64_t partition_pieces = 0; // pieces it built (over all modes)
   uint32_t *h_in_idx[GT_MAX_ORDER] = {};
   float *h_in_val = nullptr;
-  void release_host_input() {
-    for (auto &p : h_in_idx)
-      if (p) {
-        (void)hipHostFree(p);
-        p = nullptr;
+  // Host memory of the out-of-core data: page-locked where the system grants
+  // it, so the copies run asynchronously at the host link's rate; once a
+  // pinned allocation fails, ordinary pageable memory, whose copies the
+  // runtime stages (more slowly, and the host waits on them).  The pageable
+  // blocks are recorded so each is freed the way it was allocated.
+  std::unordered_map<void *, size_t> pageable, pinned;
+  uint64_t pageable_bytes = 0, pinned_bytes = 0;
+  template <typename T> void host_alloc(T **p, uint64_t count) {
+    const size_t bytes = (size_t)std::max<uint64_t>(count, 1) * sizeof(T);
+    if (o.host_pinned_limit == 0 || pinned_bytes + bytes <= o.host_pinned_limit) {
+      if (hipHostMalloc(reinterpret_cast<void **>(p), bytes, hipHostMallocDefault) ==
+          hipSuccess) {
+        pinned[*p] = bytes;
+        pinned_bytes += bytes;
+        return;
       }
-    if (h_in_val) (void)hipHostFree(h_in_val);
-    h_in_val = nullptr;
+      (void)hipGetLastError(); // a failed pinned allocation leaves no state behind
+    }
+    void *q = std::malloc(bytes);
+    if (q == nullptr)
+      throw GtError(GT_ERR_MEMORY, "host memory exhausted allocating " + std::to_string(bytes) +
+                                       " bytes for the out-of-core data");
+    pageable[q] = bytes;
+    pageable_bytes += bytes;
+    *p = static_cast<T *>(q);
+  }
+  template <typename T> void host_free(T *&p) {
+    if (p == nullptr) return;
+    const auto it = pageable.find(static_cast<void *>(p));
+    if (it != pageable.end()) {
+      pageable_bytes -= it->second;
+      pageable.erase(it);
+      std::free(static_cast<void *>(p));
+    } else {
+      const auto jt = pinned.find(static_cast<void *>(p));
+      if (jt != pinned.end()) {
+        pinned_bytes -= jt->second;
+        pinned.erase(jt);
+      }
+      (void)hipHostFree(static_cast<void *>(p));
+    }
+    p = nullptr;
+  }
+  // Moves a pageable block into page-locked memory where the system now
+  // grants it (after the partition's input copy is freed).
+  template <typename T> void pin_if_possible(T *&p) {
+    if (p == nullptr) return;
+    const auto it = pageable.find(static_cast<void *>(p));
+    if (it == pageable.end()) return;
+    const size_t bytes = it->second;
+    if (o.host_pinned_limit != 0 && pinned_bytes + bytes > o.host_pinned_limit) return;
+    void *q = nullptr;
+    if (hipHostMalloc(&q, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    std::memcpy(q, static_cast<void *>(p), bytes);
+    pageable_bytes -= bytes;
+    pageable.erase(it);
+    std::free(static_cast<void *>(p));
+    pinned[q] = bytes;
+    pinned_bytes += bytes;
+    p = static_cast<T *>(q);
+  }
+  void pin_host_streams() {
+    for (int n = 0; n < GT_MAX_ORDER; ++n) {
+      for (int k = 0; k < GT_MAX_ORDER; ++k) {
+        pin_if_possible(h_pack[n][k]);
+        pin_if_possible(h_idx[n][k]);
+      }
+      pin_if_possible(h_val[n]);
+    }
+  }
+  void release_host_input() {
+    for (auto &p : h_in_idx) host_free(p);
+    host_free(h_in_val);
     host_input = false;
   }
   // partition
@@ -303,19 +373,10 @@ struct gt_ctx {
     if (s_copy) (void)hipStreamSynchronize(s_copy);
     for (int n = 0; n < GT_MAX_ORDER; ++n) {
       for (int k = 0; k < GT_MAX_ORDER; ++k) {
-        if (h_idx[n][k]) {
-          (void)hipHostFree(h_idx[n][k]);
-          h_idx[n][k] = nullptr;
-        }
-        if (h_pack[n][k]) {
-          (void)hipHostFree(h_pack[n][k]);
-          h_pack[n][k] = nullptr;
-        }
+        host_free(h_idx[n][k]);
+        host_free(h_pack[n][k]);
       }
-      if (h_val[n]) {
-        (void)hipHostFree(h_val[n]);
-        h_val[n] = nullptr;
-      }
+      host_free(h_val[n]);
     }
     streaming = false;
     packed = false;
@@ -732,19 +793,19 @@ void maybe_stream(gt_ctx *c) {
             pack16(c->st_idx[n][k].p, nnz, tmp.p, c->s);
           else
             pack24(c->st_idx[n][k].p, nnz, tmp.p, c->s);
-          GT_HIP(hipHostMalloc(&c->h_pack[n][k], words * sizeof(uint32_t), hipHostMallocDefault));
+          c->host_alloc(&c->h_pack[n][k], words);
           GT_HIP(hipMemcpyAsync(c->h_pack[n][k], tmp.p, words * sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, c->s));
           GT_HIP(hipStreamSynchronize(c->s));
         }
       } else {
-        GT_HIP(hipHostMalloc(&c->h_idx[n][k], nnz * sizeof(uint32_t), hipHostMallocDefault));
+        c->host_alloc(&c->h_idx[n][k], nnz);
         GT_HIP(hipMemcpy(c->h_idx[n][k], c->st_idx[n][k].p, nnz * sizeof(uint32_t),
                          hipMemcpyDeviceToHost));
       }
       c->st_idx[n][k].reset();
     }
-    GT_HIP(hipHostMalloc(&c->h_val[n], nnz * sizeof(float), hipHostMallocDefault));
+    c->host_alloc(&c->h_val[n], nnz);
     GT_HIP(hipMemcpy(c->h_val[n], c->st_val[n].p, nnz * sizeof(float), hipMemcpyDeviceToHost));
     c->st_val[n].reset();
     c->where[n].reset();
@@ -1387,6 +1448,13 @@ void do_partition_host(gt_ctx *c) {
   const int N = c->N;
   const uint64_t nnz = c->nnz, budget = c->o.device_budget;
   auto t0 = std::chrono::steady_clock::now();
+  auto say = [&](const std::string &m) { // opts.verbose: one line per step
+    if (!c->o.verbose) return;
+    const double sec =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "[gt] partition under the device budget: %s (%.1f s)\n", m.c_str(), sec);
+    std::fflush(stderr);
+  };
   c->host_partitioned = true;
   c->partition_pieces = 0;
   Plan p;
@@ -1465,6 +1533,7 @@ void do_partition_host(gt_ctx *c) {
     for (int i = 0; i < g; ++i) ss += hsq[i];
   }
   c->norm = std::sqrt(ss);
+  say("counted blocks and rows over " + std::to_string((nnz + S - 1) / S) + " segments");
   std::vector<uint32_t> hcounts(c->B);
   GT_HIP(hipMemcpyAsync(hcounts.data(), counts.p, c->B * sizeof(uint32_t), hipMemcpyDeviceToHost,
                         s));
@@ -1493,12 +1562,11 @@ void do_partition_host(gt_ctx *c) {
     for (int k = 0; k < N; ++k) {
       if (packed && k == n) continue;
       if (packed)
-        GT_HIP(hipHostMalloc(&c->h_pack[n][k], pack_words(c->pack_bits[k], nnz) * sizeof(uint32_t),
-                             hipHostMallocDefault));
+        c->host_alloc(&c->h_pack[n][k], pack_words(c->pack_bits[k], nnz));
       else
-        GT_HIP(hipHostMalloc(&c->h_idx[n][k], nnz * sizeof(uint32_t), hipHostMallocDefault));
+        c->host_alloc(&c->h_idx[n][k], nnz);
     }
-    GT_HIP(hipHostMalloc(&c->h_val[n], nnz * sizeof(float), hipHostMallocDefault));
+    c->host_alloc(&c->h_val[n], nnz);
   }
 
   // ---- pass 2: per mode, bucket by bucket
@@ -1608,6 +1676,8 @@ void do_partition_host(gt_ctx *c) {
       GT_HIP(hipStreamSynchronize(s)); // the buffers are reused by the next piece
       out_pos += nb;
       ++c->partition_pieces;
+      say("mode " + std::to_string(n) + ": " + std::to_string(out_pos) + " of " +
+          std::to_string(nnz) + " nonzeros in the stream");
     };
     uint64_t r0 = 0;
     while (r0 < rows) {
@@ -1656,6 +1726,8 @@ void do_partition_host(gt_ctx *c) {
   }
   } // the partition's device buffers go here
   c->release_host_input();
+  c->pin_host_streams(); // into the page-locked memory the input held
+  say("done; " + std::to_string(c->pageable_bytes) + " bytes of the streams in pageable memory");
   c->have_coo = false;
   c->partitioned = true;
   setup_staging(c, fixed);
@@ -1859,8 +1931,7 @@ int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const floa
         if (idx[k][e] >= dims[k]) throw GtError(GT_ERR_ARG, "Coordinate is out of bounds");
       if (ctx->host_input) {
         ctx->in_idx[k].reset();
-        GT_HIP(hipHostMalloc(&ctx->h_in_idx[k], std::max<uint64_t>(nnz, 1) * sizeof(uint32_t),
-                             hipHostMallocDefault));
+        ctx->host_alloc(&ctx->h_in_idx[k], nnz);
         std::memcpy(ctx->h_in_idx[k], idx[k], nnz * sizeof(uint32_t));
         continue;
       }
@@ -1876,8 +1947,7 @@ int gt_set_coo(gt_ctx *ctx, uint64_t nnz, const uint32_t *const *idx, const floa
     }
     if (ctx->host_input) {
       ctx->in_val.reset();
-      GT_HIP(hipHostMalloc(&ctx->h_in_val, std::max<uint64_t>(nnz, 1) * sizeof(float),
-                           hipHostMallocDefault));
+      ctx->host_alloc(&ctx->h_in_val, nnz);
       std::memcpy(ctx->h_in_val, vals, nnz * sizeof(float));
     } else {
       ctx->in_val.alloc(nnz);
@@ -1928,6 +1998,7 @@ int gt_get_partition_info(gt_ctx *ctx, gt_partition_info *info) {
     info->stream_bytes_per_nnz = ctx->streaming ? ctx->stream_bytes : 0;
     info->partition_path = ctx->host_partitioned ? 1 : 0;
     info->partition_pieces = ctx->partition_pieces;
+    info->host_pageable_bytes = ctx->pageable_bytes;
   });
 }
 

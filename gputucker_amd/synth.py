@@ -19,11 +19,12 @@ def _zipf_cdf(n, s):
     return c / c[-1]
 
 
-def generate(dims, nnz, modes=None, values="uniform", seed=0, device=None):
+def generate(dims, nnz, modes=None, values="uniform", seed=0, device=None, log=None):
     """Returns (idx (N, nnz) uint32 0-based, vals float32, dims).
 
     modes: per-mode "uniform" or ("zipf", s); values: "uniform" (U[0,1)) or
-    "lognormal" (mu=0, sigma=1)."""
+    "lognormal" (mu=0, sigma=1); log: called with a line per step of the
+    bucketed form."""
     dims = [int(d) for d in dims]
     N = len(dims)
     modes = modes or ["uniform"] * N
@@ -36,19 +37,93 @@ def generate(dims, nnz, modes=None, values="uniform", seed=0, device=None):
     for k in range(N - 2, -1, -1):
         strides[k] = strides[k + 1] * dims[k + 1]
     use_torch = device is not None and str(device).startswith("cuda")
+    if use_torch and nnz > kBucketNnz:
+        return _generate_buckets(dims, nnz, modes, values, seed, device, strides, total, log)
+    return _generate_one(dims, nnz, modes, values, seed, device, strides, total)
+
+
+# Nonzeros above which the GPU generator works in buckets of mode-0 index
+# ranges: one torch sort holds at most 2^31 - 1 elements, and the candidate
+# draws are 1.25 times the target.
+kBucketNnz = 600_000_000
+
+
+def _generate_one(dims, nnz, modes, values, seed, device, strides, total, r0=None):
+    N = len(dims)
+    use_torch = device is not None and str(device).startswith("cuda")
     if total >= 2.0 ** 62:
         # two 62-bit key words: the leading modes and the trailing ones
         split = next(j for j in range(1, N) if float(np.prod([float(d) for d in dims[j:]])) < 2.0 ** 62
                      and float(np.prod([float(d) for d in dims[:j]])) < 2.0 ** 62)
         if not use_torch:
             raise ValueError("index spaces beyond 2^62 cells are generated on the GPU")
-        return _generate_torch_wide(dims, nnz, modes, values, seed, device, split)
+        return _generate_torch_wide(dims, nnz, modes, values, seed, device, split, r0)
     if use_torch:
-        return _generate_torch(dims, nnz, modes, values, seed, device, strides)
-    return _generate_numpy(dims, nnz, modes, values, seed, strides)
+        return _generate_torch(dims, nnz, modes, values, seed, device, strides, r0)
+    return _generate_numpy(dims, nnz, modes, values, seed, strides, r0)
 
 
-def _generate_numpy(dims, nnz, modes, values, seed, strides):
+def _generate_buckets(dims, nnz, modes, values, seed, device, strides, total, log=None):
+    """Large targets: mode 0's range is cut into K contiguous pieces of equal
+    probability mass, each piece gets a multinomial share of the nonzeros and
+    its own seeded draw with mode 0 restricted to the piece (distinct pieces
+    never share a coordinate, so each piece's dedup is the global one), and
+    the pieces are merged in a seeded random order: slice c of every piece
+    forms chunk c, which is shuffled as a whole."""
+    import torch
+    N = len(dims)
+    K = -(-nnz // (kBucketNnz // 2))
+    m0 = modes[0]
+    if m0 == "uniform":
+        cdf = np.arange(1, dims[0] + 1, dtype=np.float64) / dims[0]
+    else:
+        cdf = _zipf_cdf(dims[0], m0[1])
+    cuts = [0] + [int(np.searchsorted(cdf, (b + 1) / K, side="left")) + 1 for b in range(K - 1)]
+    cuts = sorted(set(min(max(c, 1), dims[0] - 1) for c in cuts[1:]))
+    cuts = [0] + cuts + [dims[0]]
+    K = len(cuts) - 1
+    mass = np.array([cdf[cuts[b + 1] - 1] - (cdf[cuts[b] - 1] if cuts[b] > 0 else 0.0)
+                     for b in range(K)])
+    rng = np.random.default_rng(seed + 104729)
+    counts = rng.multinomial(nnz, mass / mass.sum())
+    pieces = []
+    for b in range(K):
+        cells = float(cuts[b + 1] - cuts[b]) * total / dims[0]
+        if counts[b] > cells:
+            raise ValueError("a mode-0 piece has fewer cells than its share of nonzeros")
+        ib, vb, _ = _generate_one(dims, int(counts[b]), modes, values, seed * 1009 + 17 * b + 1,
+                                  device, strides, total, (cuts[b], cuts[b + 1]))
+        pieces.append((ib, vb))
+        if log:
+            log(f"piece {b + 1}/{K}: mode-0 rows [{cuts[b]}, {cuts[b + 1]}), {counts[b]} nonzeros")
+    C = K
+    idx = np.empty((N, nnz), np.uint32)
+    vals = np.empty(nnz, np.float32)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed) + 7)
+    pos = 0
+    for c in range(C):
+        parts_i, parts_v = [], []
+        for ib, vb in pieces:
+            n = ib.shape[1]
+            lo, hi = n * c // C, n * (c + 1) // C
+            parts_i.append(ib[:, lo:hi])
+            parts_v.append(vb[lo:hi])
+        ci = torch.from_numpy(np.concatenate(parts_i, axis=1).view(np.int32)).to(device)
+        cv = torch.from_numpy(np.concatenate(parts_v)).to(device)
+        n = cv.shape[0]
+        perm = torch.randperm(n, generator=g, device=device)
+        idx[:, pos:pos + n] = ci[:, perm].cpu().numpy().view(np.uint32)
+        vals[pos:pos + n] = cv[perm].cpu().numpy()
+        pos += n
+        del ci, cv, perm
+        if log:
+            log(f"merged chunk {c + 1}/{C}")
+    torch.cuda.empty_cache()
+    return idx, vals, dims
+
+
+def _generate_numpy(dims, nnz, modes, values, seed, strides, r0=None):
     rng = np.random.default_rng(seed)
     N = len(dims)
     cdfs = [None if m == "uniform" else _zipf_cdf(dims[k], m[1]) for k, m in enumerate(modes)]
@@ -57,10 +132,13 @@ def _generate_numpy(dims, nnz, modes, values, seed, strides):
         m = int((nnz - len(keys)) * 1.25) + 1024
         key = np.zeros(m, np.int64)
         for k in range(N):
+            lo, hi = (r0 if k == 0 and r0 is not None else (0, dims[k]))
             if cdfs[k] is None:
-                col = rng.integers(0, dims[k], m)
+                col = rng.integers(lo, hi, m)
             else:
-                col = np.minimum(np.searchsorted(cdfs[k], rng.random(m), side="right"), dims[k] - 1)
+                base = cdfs[k][lo - 1] if lo > 0 else 0.0
+                u = base + (cdfs[k][hi - 1] - base) * rng.random(m)
+                col = np.clip(np.searchsorted(cdfs[k], u, side="right"), lo, hi - 1)
             key += col.astype(np.int64) * strides[k]
         keys = np.unique(np.concatenate([keys, key]))
     keys = keys[rng.permutation(len(keys))[:nnz]]
@@ -71,7 +149,19 @@ def _generate_numpy(dims, nnz, modes, values, seed, strides):
     return idx, vals.astype(np.float32), dims
 
 
-def _generate_torch(dims, nnz, modes, values, seed, device, strides):
+def _draw_col(torch, k, m, dims, cdfs, g, device, r0):
+    """m draws of mode k; mode 0 restricted to the index range r0 = (lo, hi)."""
+    lo, hi = (r0 if k == 0 and r0 is not None else (0, dims[k]))
+    if cdfs[k] is None:
+        return torch.randint(lo, hi, (m,), generator=g, device=device)
+    c = cdfs[k]
+    base = float(c[lo - 1]) if lo > 0 else 0.0
+    top = float(c[hi - 1])
+    u = base + (top - base) * torch.rand(m, generator=g, device=device, dtype=torch.float64)
+    return torch.searchsorted(c, u, right=True).clamp_(min=lo, max=hi - 1)
+
+
+def _generate_torch(dims, nnz, modes, values, seed, device, strides, r0=None):
     import torch
     g = torch.Generator(device=device)
     g.manual_seed(int(seed))
@@ -83,11 +173,7 @@ def _generate_torch(dims, nnz, modes, values, seed, device, strides):
         m = int((nnz - keys.numel()) * 1.25) + 1024
         key = torch.zeros(m, dtype=torch.int64, device=device)
         for k in range(N):
-            if cdfs[k] is None:
-                col = torch.randint(0, dims[k], (m,), generator=g, device=device)
-            else:
-                u = torch.rand(m, generator=g, device=device, dtype=torch.float64)
-                col = torch.searchsorted(cdfs[k], u, right=True).clamp_(max=dims[k] - 1)
+            col = _draw_col(torch, k, m, dims, cdfs, g, device, r0)
             key += col.to(torch.int64) * strides[k]
             del col
         keys = torch.unique(torch.cat([keys, key]))
@@ -108,7 +194,7 @@ def _generate_torch(dims, nnz, modes, values, seed, device, strides):
     return idx, vals, dims
 
 
-def _generate_torch_wide(dims, nnz, modes, values, seed, device, split):
+def _generate_torch_wide(dims, nnz, modes, values, seed, device, split, r0=None):
     """As _generate_torch, for index spaces of 2^62 cells and more: a
     coordinate is two key words (modes [0, split) and [split, N)), and
     duplicates are found after a lexicographic sort of the word pairs (stable
@@ -136,11 +222,7 @@ def _generate_torch_wide(dims, nnz, modes, values, seed, device, split):
         for grp, st in zip(groups, gstr):
             w = torch.zeros(m, dtype=torch.int64, device=device)
             for k in grp:
-                if cdfs[k] is None:
-                    col = torch.randint(0, dims[k], (m,), generator=g, device=device)
-                else:
-                    u = torch.rand(m, generator=g, device=device, dtype=torch.float64)
-                    col = torch.searchsorted(cdfs[k], u, right=True).clamp_(max=dims[k] - 1)
+                col = _draw_col(torch, k, m, dims, cdfs, g, device, r0)
                 w += col.to(torch.int64) * st[k]
                 del col
             words.append(w)

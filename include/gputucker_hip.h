@@ -102,6 +102,11 @@ typedef struct gt_opts {
                                        from its default source; 1: from the other one (a table
                                        in LDS or L2) where both are built.  gt_autotune
                                        chooses per mode instead (tuned_variant) */
+  uint64_t host_pinned_limit;       /* out-of-core host data: at most this many page-locked
+                                       bytes (0: as many as the system grants); the rest in
+                                       pageable memory (gt_partition_info.host_pageable_bytes) */
+  int verbose;                      /* 1: a line on stderr per step of a partition built under
+                                       the device budget (each pass over the input) */
 } gt_opts;
 
 typedef struct gt_partition_info {
@@ -128,6 +133,9 @@ typedef struct gt_partition_info {
                                 stream was built piece by piece (ranges of rows, of blocks inside
                                 a heavy row, windows inside a heavy block) under the budget */
   uint64_t partition_pieces; /* pieces the bucketed partition built, over all modes */
+  uint64_t host_pageable_bytes; /* out-of-core host data held in pageable memory because the
+                                   system granted no more page-locked memory (0 normally);
+                                   copies from it are slower and synchronous */
 } gt_partition_info;
 
 typedef struct gt_timings {

@@ -708,3 +708,30 @@ def test_host_partition_windows_inside_a_heavy_block():
     f_ref, f_oc = ref.run(), oc.run()
     for m in range(5):
         assert np.array_equal(ref.get_factor(m), oc.get_factor(m)), m
+
+
+@pytest.mark.parametrize("limit", [1, 700_000])
+def test_out_of_core_pageable_host_memory(limit):
+    """When the system grants no more page-locked memory, the out-of-core
+    host data (the waiting input, the per-mode streams) lives in pageable
+    memory instead; host_pinned_limit forces that here (1 byte: all of it;
+    700 kB: the input and the first streams pinned, the rest pageable).  Copies from pageable
+    memory go through the runtime's staging, and the factors must still be
+    those of the in-core run, bit for bit."""
+    print(f"out of core, pageable host memory past {limit} pinned bytes", flush=True)
+    idx, vals, dims = random_coo(11, [300, 200, 100], 30000, skew=1.3)
+    A, G = random_init(11, dims, 6)
+    kw = dict(precision="bf16r", chunk_tiles=1, max_iters=2, tol=0.0)
+    ref = make(idx, vals, dims, 6, [2, 1, 2], **kw)
+    ref.set_init(A, G)
+    oc = make(idx, vals, dims, 6, [2, 1, 2], device_budget=1_500_000, host_pinned_limit=limit,
+              **kw)
+    oc.set_init(A, G)
+    info = oc.info()
+    assert info["out_of_core"] == 1 and info["partition_path"] == 1, info
+    # streams: 30000 nonzeros x (two 16-bit indices + a value) x 3 modes
+    assert 0 < info["host_pageable_bytes"] <= 3 * 30000 * 8 + 4096, info
+    f_ref, f_oc = ref.run(), oc.run()
+    for m in range(3):
+        assert np.array_equal(ref.get_factor(m), oc.get_factor(m)), m
+    assert np.allclose(f_ref, f_oc, rtol=0, atol=1e-12), (f_ref, f_oc)
