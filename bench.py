@@ -273,6 +273,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--nnz", type=float, default=None,
                     help="nonzeros of the workload (default: the config's)")
+    ap.add_argument("--rank", type=int, default=None,
+                    help="Tucker rank per mode (default: the config's)")
     ap.add_argument("--no-graph", dest="graph", action="store_false",
                     help="launch each iteration's kernels eagerly instead of replaying the "
                          "captured HIP graph")
@@ -288,6 +290,9 @@ def main(argv=None):
     if a.nnz is not None:
         cfg["nnz"] = int(a.nnz)
         cfg["name"] = re.sub(r"[0-9.e]+ nnz", f"{cfg['nnz']:.3g} nnz", cfg["name"])
+    if a.rank is not None:
+        cfg["rank"] = int(a.rank)
+        cfg["name"] = re.sub(r"R=\d+", f"R={cfg['rank']}", cfg["name"])
     N, R = len(cfg["dims"]), cfg["rank"]
     if R > 15 and a.precision == "bf16":
         # plain BF16 is built for R <= 15; the wide ranks have the refined delta
