@@ -192,47 +192,65 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wave per row (16 < R <= 64).  Lane i < R holds row i of B in
-// registers and turns it into row i of the Cholesky factor L, right-looking:
-// at step j the diagonal comes from lane j, each lane below scales its
-// L(i, j), column j goes through LDS to every lane, and each lane updates
-// its own row (M(i, k) -= L(i, j) L(k, j), k > j).  Every step is unrolled,
-// so the row stays in registers.  The forward solve L y = c then runs on the
-// rows in registers (y_k from lane k to all), and L goes to LDS for the
-// backward solve L^T a = y (row k of L read by the lanes of its columns) and
-// the fit partial.  Lanes at and past R shadow row R - 1; their results are
-// never used.  Above the diagonal the registers hold values no step reads.
+// floor(64 / R) rows per wave (16 < R <= kSolveRegMax: three up to R = 21,
+// two up to R = 32): lane group gi (lanes gi R .. gi R + R - 1) solves row
+// NR * block + gi.  Lane i of a group holds row i of B in registers and
+// turns it into row i of the Cholesky factor L, right-looking: at step j the
+// diagonal comes from the group's lane j, each lane below scales its L(i, j),
+// column j goes through LDS to the group's lanes, and each lane updates its
+// own row (M(i, k) -= L(i, j) L(k, j), k > j).  Every step is unrolled, so
+// the row stays in registers.  The forward solve L y = c then runs on the
+// rows in registers (y_k from the group's lane k), and L goes to LDS (packed
+// lower triangle) for the backward solve L^T a = y (row k of L read by the
+// lanes of its columns) and the fit partial.  Lanes past the last group
+// shadow its row R - 1, and groups past the last row (or on an empty row)
+// solve the identity; their results are never stored.  Above the diagonal
+// the registers hold values no step reads.
+template <int R> constexpr int solve_rows_per_wave() { return kWave / R; }
+
+// The value of v in lane `base + k` of the wave, for every lane (base may
+// differ between lanes: one read per lane group); with one group, a readlane.
+template <int NR> __device__ inline double group_read(double v, int base, int k) {
+  if constexpr (NR == 1) {
+    return readlane_f64(v, k);
+  } else {
+    return __shfl(v, base + k);
+  }
+}
+
 template <int R>
 __global__ void __launch_bounds__(64) k_solve_wave(SolveArgs a) {
   constexpr int W = gram_width(R);
-  constexpr int LP = R + 1; // pitch of L in LDS (doubles)
-  __shared__ double colj[R];
-  __shared__ double Lm[R * LP];
-  const uint64_t row = blockIdx.x;
+  constexpr int NR = solve_rows_per_wave<R>();
+  constexpr int LT = R * (R + 1) / 2; // L(k, i) at k (k + 1) / 2 + i
+  __shared__ double colj[NR][R];
+  __shared__ double Lm[NR][LT];
+  __shared__ double part[kWave];
   const int lane = threadIdx.x;
-  float *out = a.A + row * row_stride(R);
-  if (a.row_ptr[row + 1] == a.row_ptr[row]) {
-    if (lane < R) out[lane] = 0.f;
-    if (a.fit_partials && lane == 0) a.fit_partials[row] = 0.0;
-    return;
-  }
-  const double *g = a.gram + row * W;
-  const int li = lane < R ? lane : R - 1;
+  const bool act = lane / R < NR; // a group's lane, not a shadow
+  const int gi = act ? lane / R : NR - 1;
+  const int li = act ? lane % R : R - 1;
+  const int base = gi * R;
+  const uint64_t row = (uint64_t)blockIdx.x * NR + gi;
+  const bool live = row < a.rows;
+  const uint64_t rr = live ? row : a.rows - 1; // an in-bounds row for the reads
+  const bool solve = live && a.row_ptr[rr + 1] != a.row_ptr[rr];
+  const double *g = a.gram + rr * W;
   double m[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    const double v = g[k <= li ? up_index(R, k, li) : 0];
-    m[k] = k < li ? v : (k == li ? v + a.lambda : 0.0);
+    const double v = solve ? g[k <= li ? up_index(R, k, li) : 0] : 0.0;
+    m[k] = k < li ? v : (k == li ? (solve ? v + a.lambda : 1.0) : 0.0);
   }
-  const double ci = g[R * (R + 1) / 2 + li];
+  const double ci = solve ? g[R * (R + 1) / 2 + li] : 0.0;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    const double d = sqrt(readlane_f64(m[j], j));
+    const double d = sqrt(group_read<NR>(m[j], base, j));
     m[j] = li == j ? d : (li > j ? m[j] / d : m[j]);
-    colj[li] = m[j];
+    colj[gi][li] = m[j]; // a shadow lane writes what its group's lane R - 1 writes
     wave_lds_sync();
 #pragma unroll
-    for (int k = j + 1; k < R; ++k) m[k] -= m[j] * colj[k];
+    for (int k = j + 1; k < R; ++k) m[k] -= m[j] * colj[gi][k];
     wave_lds_sync(); // every read of column j before column j + 1 is written
   }
   // forward: y_k = (c_k - sum_{i<k} L(k, i) y_i) / L(k, k), lane k's c
@@ -240,14 +258,14 @@ __global__ void __launch_bounds__(64) k_solve_wave(SolveArgs a) {
   double c = ci, y = 0.0;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    const double yk = readlane_f64(c, k) / readlane_f64(m[k], k);
+    const double yk = group_read<NR>(c, base, k) / group_read<NR>(m[k], base, k);
     if (li == k) y = yk;
     c -= m[k] * yk;
   }
-  // L to LDS, row i at i * LP
+  // L to LDS, row i at i (i + 1) / 2
 #pragma unroll
   for (int k = 0; k < R; ++k)
-    if (k <= li) Lm[li * LP + k] = m[k];
+    if (k <= li) Lm[gi][li * (li + 1) / 2 + k] = m[k];
   wave_lds_sync();
   // backward: a_k = (y_k - sum_{i>k} L(i, k) a_i) / L(k, k), descending k;
   // lane i < k subtracts L(k, i) a_k from its y.  With the fit partial, lane i
@@ -256,23 +274,28 @@ __global__ void __launch_bounds__(64) k_solve_wave(SolveArgs a) {
   float af = 0.f;
 #pragma unroll
   for (int k = R - 1; k >= 0; --k) {
-    const double ak = readlane_f64(y, k) / Lm[k * LP + k];
+    const double ak = group_read<NR>(y, base, k) / Lm[gi][k * (k + 1) / 2 + k];
     const float akf = (float)ak;
     if (li == k) af = akf;
-    const double lki = Lm[k * LP + (li <= k ? li : k)];
+    const double lki = Lm[gi][k * (k + 1) / 2 + (li <= k ? li : k)];
     if (li < k) y -= lki * ak;
     if (li <= k) v += lki * (double)akf;
   }
-  if (lane < R) out[lane] = af;
-  if (a.fit_partials) { // a^T (L L^T) a - lambda |a|^2 - 2 a^T c
+  if (act && live) a.A[row * row_stride(R) + li] = solve ? af : 0.f;
+  if (a.fit_partials) { // a^T (L L^T) a - lambda |a|^2 - 2 a^T c, summed over the group
     double contrib = 0.0;
-    if (lane < R) {
+    if (act && solve) {
       const double ai = (double)af;
       contrib = v * v - a.lambda * ai * ai - 2.0 * ai * ci;
     }
+    part[lane] = contrib;
+    wave_lds_sync();
+    if (act && live && li == 0) {
+      double s = 0.0;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) contrib += __shfl_xor(contrib, off);
-    if (lane == 0) a.fit_partials[row] = contrib;
+      for (int i = 0; i < R; ++i) s += part[base + i];
+      a.fit_partials[row] = s;
+    }
   }
 }
 
@@ -343,7 +366,9 @@ template <int R> uint64_t solve_impl(const SolveArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_solve_lane<R>, dim3((unsigned)grid), dim3(64), 0, s, a);
     return grid;
   } else if constexpr (R <= kSolveRegMax) {
-    hipLaunchKernelGGL(k_solve_wave<R>, dim3((unsigned)a.rows), dim3(64), 0, s, a);
+    constexpr int NR = solve_rows_per_wave<R>();
+    const uint64_t grid = (a.rows + NR - 1) / NR;
+    hipLaunchKernelGGL(k_solve_wave<R>, dim3((unsigned)grid), dim3(64), 0, s, a);
     return a.rows;
   } else {
     hipLaunchKernelGGL(k_solve_wave_lds<R>, dim3((unsigned)a.rows), dim3(64), 0, s, a);
