@@ -614,23 +614,37 @@ std::vector<uint16_t> bf16_core_table(const std::vector<float> &gt, int N, int R
 
 // The core operand table of the wide refined BF16 delta (order 3): entry
 // (o, mt, lane (q, g)) holds the BF16 hi words of G[o][8g + t][16 mt + q],
-// t < 8, then the lo words bf16(G - hi), zero at jL or r >= R.
+// t < 8, then the lo words bf16(G - hi), zero at jL or r >= R, for the
+// wide_tiles_per_o(R) tiles formed per outer index.  With the last tile
+// packed (PO = wide_pack(R) > 1 outer indices per MFMA), entry (ob, lane
+// (q, g)) then holds the words of G[PO ob + q / (16 / PO)][8g + t][16 (MT - 1)
+// + q % (16 / PO)], zero past R.
 std::vector<uint16_t> bf16_core_table_wide(const std::vector<float> &gt, int R) {
-  const int MT = (R + 15) / 16;
+  const int MT = wide_tiles(R), MTO = wide_tiles_per_o(R), PO = wide_pack(R);
   std::vector<uint16_t> tab(gb_wide_words(R), 0);
+  auto put = [&](uint16_t *dst, int o, int g, int r) {
+    for (int t = 0; t < 8; ++t) {
+      const int jL = 8 * g + t;
+      const float v = (o < R && jL < R && r < R) ? gt[((size_t)o * R + jL) * R + r] : 0.f;
+      const uint16_t h = to_bf16(v);
+      dst[t] = h;
+      dst[8 + t] = to_bf16(v - from_bf16(h));
+    }
+  };
   for (int o = 0; o < R; ++o)
-    for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < MTO; ++mt)
+      for (int l = 0; l < kWave; ++l)
+        put(tab.data() + (((size_t)o * MTO + mt) * kWave + l) * 16, o, l / 16, 16 * mt + l % 16);
+  if (PO > 1) {
+    const size_t base = (size_t)R * MTO * kWave * 16;
+    const int RPr = 16 / PO;
+    for (int ob = 0; ob < wide_packed_blocks(R); ++ob)
       for (int l = 0; l < kWave; ++l) {
-        const int q = l % 16, g = l / 16, r = 16 * mt + q;
-        uint16_t *dst = tab.data() + (((size_t)o * MT + mt) * kWave + l) * 16;
-        for (int t = 0; t < 8; ++t) {
-          const int jL = 8 * g + t;
-          const float v = (jL < R && r < R) ? gt[((size_t)o * R + jL) * R + r] : 0.f;
-          const uint16_t h = to_bf16(v);
-          dst[t] = h;
-          dst[8 + t] = to_bf16(v - from_bf16(h));
-        }
+        const int q = l % 16;
+        put(tab.data() + base + ((size_t)ob * kWave + l) * 16, PO * ob + q / RPr, l / 16,
+            16 * (MT - 1) + q % RPr);
       }
+  }
   return tab;
 }
 

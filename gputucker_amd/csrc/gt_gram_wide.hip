@@ -32,7 +32,9 @@ typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 
 template <int R> struct WideLayout {
-  static constexpr int MT = (R + 15) / 16;           // 16-column output tiles
+  static constexpr int MT = wide_tiles(R);           // 16-column output tiles
+  static constexpr int PO = wide_pack(R);            // outer indices per packed last tile
+  static constexpr int MTO = wide_tiles_per_o(R);    // tiles formed per outer index
   static constexpr int RP = ((R + 1) + 3) / 4 * 4;   // delta tile pitch: 16-byte rows, x at R
   static constexpr int S_FLOATS = R * kWave;         // s values, [o][nonzero]
   static constexpr int T_FLOATS = kWave * RP;        // delta tile, [nonzero][column]
@@ -45,7 +47,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_gram_bf16w(GramArgs a, int wpb) {
   static_assert(R >= 16 && R <= 32, "the wide kernel covers 16 <= R <= 32");
   using Lay = WideLayout<R>;
-  constexpr int MT = Lay::MT, RP = Lay::RP, RS = row_stride(R);
+  constexpr int MT = Lay::MT, MTO = Lay::MTO, PO = Lay::PO, RP = Lay::RP, RS = row_stride(R);
+  constexpr int RPr = 16 / PO; // rows per outer index in the packed last tile
   extern __shared__ __align__(16) float dyn[];
   const int lane = threadIdx.x & (kWave - 1), w = wave_index();
   const uint64_t c = a.chunk_begin + (uint64_t)blockIdx.x * wpb + w;
@@ -55,8 +58,13 @@ k_gram_bf16w(GramArgs a, int wpb) {
   const uint64_t cb = c * a.chunk;
   const uint64_t ce = (cb + a.chunk < a.st.nnz) ? cb + a.chunk : a.st.nnz;
   const int q = lane & 15, g = lane >> 4;
-  // core operand words, [o][mt][lane][hi, lo] (16 bytes each)
+  // core operand words, [o][mt][lane][hi, lo] (16 bytes each), then the
+  // packed last tile's [ob][lane][hi, lo]
   const u4 *tab = reinterpret_cast<const u4 *>(a.Gb);
+  const u4 *tab_p = tab + (size_t)R * MTO * kWave * 2;
+  // lane (q, g) of a packed tile holds rows 4g + i: outer index PO ob + o_sub,
+  // column 16 (MT - 1) + c_sub + i
+  const int o_sub = (4 * g) / RPr, c_sub = (4 * g) % RPr;
   BlockGram<R, RP> gram(a, c, cb, ce, lane);
 
   for (uint64_t tb = cb; tb < ce; tb += kWave) {
@@ -85,39 +93,48 @@ k_gram_bf16w(GramArgs a, int wpb) {
       for (int o = 0; o < R; ++o) buf[o * kWave + lane] = ro[o];
     }
     s_row[lane] = valid ? row : 0xFFFFFFFFu;
-    u4 nh[MT], nl[MT];
+    u4 nh[MTO], nl[MTO];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
+    for (int mt = 0; mt < MTO; ++mt) {
       nh[mt] = tab[((size_t)mt * kWave + lane) * 2];
       nl[mt] = tab[((size_t)mt * kWave + lane) * 2 + 1];
     }
     wave_sync();
-    f4 dacc[4][MT];
+    f4 dacc[4][MT]; // [mb][MT - 1]: the packed last tile, when PO > 1
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) dacc[mb][mt] = f4{0.f, 0.f, 0.f, 0.f};
+    u4 ph = u4{0u, 0u, 0u, 0u}, pl = ph; // the packed tile's core operand of this block
 #pragma unroll 1
     for (int o = 0; o < R; ++o) {
-      u4 gh[MT], gl[MT];
+      u4 gh[MTO], gl[MTO];
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
+      for (int mt = 0; mt < MTO; ++mt) {
         gh[mt] = nh[mt];
         gl[mt] = nl[mt];
       }
       if (o + 1 < R) { // the next outer index's core operand, from L2
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          nh[mt] = tab[(((size_t)(o + 1) * MT + mt) * kWave + lane) * 2];
-          nl[mt] = tab[(((size_t)(o + 1) * MT + mt) * kWave + lane) * 2 + 1];
+        for (int mt = 0; mt < MTO; ++mt) {
+          nh[mt] = tab[(((size_t)(o + 1) * MTO + mt) * kWave + lane) * 2];
+          nl[mt] = tab[(((size_t)(o + 1) * MTO + mt) * kWave + lane) * 2 + 1];
         }
+      }
+      // the packed tile of block o / PO: its operand is loaded at the block's
+      // first outer index and used at its last
+      const bool p_first = PO > 1 && o % PO == 0;
+      const bool p_last = PO > 1 && (o % PO == PO - 1 || o == R - 1);
+      if (p_first) {
+        ph = tab_p[((size_t)(o / PO) * kWave + lane) * 2];
+        pl = tab_p[((size_t)(o / PO) * kWave + lane) * 2 + 1];
       }
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
         const float sv = buf[o * kWave + mb * 16 + q];
         const b8 bh = __builtin_bit_cast(b8, ah[mb]), bl = __builtin_bit_cast(b8, al[mb]);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
+        for (int mt = 0; mt < MTO; ++mt) {
           const b8 th = __builtin_bit_cast(b8, gh[mt]), tl = __builtin_bit_cast(b8, gl[mt]);
           f4 u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(th, bh, (f4{0.f, 0.f, 0.f, 0.f}), 0, 0, 0);
           u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(th, bl, u, 0, 0, 0);
@@ -125,7 +142,32 @@ k_gram_bf16w(GramArgs a, int wpb) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) dacc[mb][mt][i] = __builtin_fmaf(u[i], sv, dacc[mb][mt][i]);
         }
+        if constexpr (PO > 1) {
+          if (p_last) { // (a uniform branch: MFMAs never sit under a lane mask)
+            const b8 th = __builtin_bit_cast(b8, ph), tl = __builtin_bit_cast(b8, pl);
+            f4 u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(th, bh, (f4{0.f, 0.f, 0.f, 0.f}), 0, 0,
+                                                           0);
+            u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(th, bl, u, 0, 0, 0);
+            u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(tl, bh, u, 0, 0, 0);
+            // this lane group's rows are outer index PO (o / PO) + o_sub (past R:
+            // zero rows of the table, any s)
+            const int op = PO * (o / PO) + o_sub;
+            const float spv = buf[(op < R ? op : R - 1) * kWave + mb * 16 + q];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              dacc[mb][MT - 1][i] = __builtin_fmaf(u[i], spv, dacc[mb][MT - 1][i]);
+          }
+        }
       }
+    }
+    if constexpr (PO > 1) { // the packed tile: add the lane groups of the same columns
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if constexpr (PO == 4) dacc[mb][MT - 1][i] = add_xor16(dacc[mb][MT - 1][i]);
+          dacc[mb][MT - 1][i] = add_xor32(dacc[mb][MT - 1][i]);
+        }
     }
     wave_sync(); // every s value read before the delta tile overwrites them
     // lane (q, g) holds delta[16mb + q][16mt + 4g + i]: four columns per
@@ -134,9 +176,13 @@ k_gram_bf16w(GramArgs a, int wpb) {
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-        if (16 * mt + 4 * g < R)
-          *reinterpret_cast<f4 *>(tile + (mb * 16 + q) * RP + 16 * mt + 4 * g) = dacc[mb][mt];
+      for (int mt = 0; mt < MT; ++mt) {
+        // the packed tile's lane groups past the first outer index hold copies
+        const int col = 16 * mt + (PO > 1 && mt == MT - 1 ? c_sub : 4 * g);
+        const bool own = PO > 1 && mt == MT - 1 ? o_sub == 0 : true;
+        if (own && col < R)
+          *reinterpret_cast<f4 *>(tile + (mb * 16 + q) * RP + col) = dacc[mb][mt];
+      }
     tile[lane * RP + R] = x;
     wave_sync();
     if (a.delta_out != nullptr && lane < cnt) { // test hook, uniform branch

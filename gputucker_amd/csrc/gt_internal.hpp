@@ -290,13 +290,27 @@ void launch_split_factor(int R, const float *A, uint64_t rows, uint32_t *As, hip
 // built for R = 16, 20, 24, 28, 30, 32) with the block-per-lane FP64 Gram.
 // It gathers a wide split copy of A_L (kSplitWordsWide words per row: per
 // lane group g, hi then lo of columns 8g..8g+7) and reads its core operand,
-// [o][mt][lane][hi 8, lo 8] BF16 words, from global memory.
+// [o][mt][lane][hi 8, lo 8] BF16 words for the tiles formed per outer index,
+// then [ob][lane][hi 8, lo 8] for the packed last tile, from global memory.
 constexpr int kSplitWordsWide = 32;
 bool gram_wide_supported(int R, int order);
 void launch_gram_bf16_wide(int R, const GramArgs &a, hipStream_t s);
 void launch_split_wide(int R, const float *A, uint64_t rows, uint32_t *As, hipStream_t s);
+// The last 16-column output tile of the wide kernel holds RR = R - 16 (MT - 1)
+// columns; when RR <= 8 it packs wide_pack(R) outer indices (rows (o - PO ob,
+// r) with r padded to 16 / PO), one MFMA set per PO outer indices.
+__host__ __device__ constexpr int wide_tiles(int R) { return (R + 15) / 16; }
+__host__ __device__ constexpr int wide_pack(int R) {
+  return R - 16 * (wide_tiles(R) - 1) <= 4 ? 4 : R - 16 * (wide_tiles(R) - 1) <= 8 ? 2 : 1;
+}
+__host__ __device__ constexpr int wide_tiles_per_o(int R) { // tiles formed per outer index
+  return wide_pack(R) == 1 ? wide_tiles(R) : wide_tiles(R) - 1;
+}
+__host__ __device__ constexpr int wide_packed_blocks(int R) { // packed last-tile blocks
+  return wide_pack(R) == 1 ? 0 : (R + wide_pack(R) - 1) / wide_pack(R);
+}
 __host__ __device__ constexpr size_t gb_wide_words(int R) { // uint16 words of the core table
-  return (size_t)R * ((R + 15) / 16) * kWave * 16;
+  return ((size_t)R * wide_tiles_per_o(R) + wide_packed_blocks(R)) * kWave * 16;
 }
 
 struct SolveArgs {
