@@ -1,6 +1,7 @@
-"""The benchmark workload at full size (3-order 1e6^3, 1e8 nonzeros with
-Zipf(1.1) coordinates, R = 10, P = 4x4x4), where the oracle is too slow to
-run: properties that hold at any size.
+"""The benchmark workloads at full size (order 3 1e6^3 with 1e8 Zipf(1.1)
+nonzeros at R = 10; order 4 with 2e8 nonzeros at R = 8; order 3 with 3e8
+nonzeros at R = 32; order 5 with 2e8 nonzeros at R = 5), where the oracle
+is too slow to run: properties that hold at any size.
 
 - Each mode update minimises every row's regularised least-squares problem
   exactly, so the ALS objective sum_e (x_e - x^_e)^2 + lambda sum_n |A_n|^2
@@ -13,40 +14,47 @@ The smaller parity tests compare the same kernels with the oracle directly."""
 import numpy as np
 import pytest
 
+import bench
 import gputucker_amd as gt
 
 pytestmark = pytest.mark.gpu
 
-R = 10
 
-
-def objective(e, norm, lam):
+def objective(e, N, norm, lam):
     """sum_e (x_e - x^_e)^2 + lambda sum_n |A_n|^2, with the error recovered
     from the reconstruction fit 1 - sqrt(err) / |x|."""
     err = ((1.0 - e.fit()) * norm) ** 2
-    reg = sum(float(np.square(e.get_factor(n).astype(np.float64)).sum()) for n in range(3))
+    reg = sum(float(np.square(e.get_factor(n).astype(np.float64)).sum()) for n in range(N))
     return err + lam * reg
 
 
-def test_benchmark_workload_properties():
+# the BF16-refined and FP32 fits after one mode sweep and one iteration,
+# relative (measured: 6e-9, 2e-9, 5e-8 and 3e-11)
+FIT_AGREEMENT = {"c2": 1e-6, "c3": 1e-6, "c4": 1e-6, "c5s": 1e-6}
+
+
+@pytest.mark.parametrize("config", ["c2", "c3", "c4", "c5s"])
+def test_benchmark_workload_properties(config):
     from gputucker_amd import synth
-    print("generating 1e8 nonzeros", flush=True)
-    idx, vals, dims = synth.generate([1_000_000] * 3, 100_000_000, [("zipf", 1.1)] * 3,
-                                     "uniform", seed=0, device="cuda")
+    cfg = bench.CONFIGS[config]
+    N, R = len(cfg["dims"]), cfg["rank"]
+    print(f"{config}: generating {cfg['name']}", flush=True)
+    idx, vals, dims = synth.generate(cfg["dims"], cfg["nnz"], cfg["modes"], cfg["values"],
+                                     seed=0, device="cuda")
     A, G = synth.init_factors(dims, R, seed=0)
     lam = gt.default_lambda()
     norm = float(np.sqrt(np.square(vals.astype(np.float64)).sum()))
     fits = {}
     for prec in ("bf16r", "fp32"):
-        print(f"precision {prec}", flush=True)
-        with gt.Engine(3, R, partition=[4, 4, 4], lam=lam, precision=prec) as e:
+        print(f"{config}: precision {prec}", flush=True)
+        with gt.Engine(N, R, partition=cfg["part"], lam=lam, precision=prec) as e:
             e.set_coo(idx, vals, dims)
             e.partition()
             e.set_init(A, G)
-            prev = objective(e, norm, lam)
-            for n in range(3):
+            prev = objective(e, N, norm, lam)
+            for n in range(N):
                 e.mode_update(n)
-                obj = objective(e, norm, lam)
+                obj = objective(e, N, norm, lam)
                 # exact minimisation per row; FP32 factors and rounded deltas
                 # leave at most rounding-level slack
                 assert obj <= prev * (1 + 1e-6), (prec, n, prev, obj)
@@ -56,4 +64,6 @@ def test_benchmark_workload_properties():
             assert np.isfinite(f_iter) and 0.0 < f_recon < 1.0
             assert abs(f_iter - f_recon) <= 2e-5 * abs(f_recon), (prec, f_iter, f_recon)
             fits[prec] = f_recon
-    assert abs(fits["bf16r"] - fits["fp32"]) <= 1e-6 * abs(fits["fp32"]), fits
+    rel = abs(fits["bf16r"] - fits["fp32"]) / abs(fits["fp32"])
+    print(f"{config}: fits {fits}, relative difference {rel:.2e}", flush=True)
+    assert rel <= FIT_AGREEMENT[config], fits
