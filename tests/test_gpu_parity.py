@@ -110,7 +110,7 @@ def _stream_positions(t, idx_stream):
 @pytest.mark.parametrize("N,R,part", [(3, 10, [2, 1, 2]), (3, 13, [1, 1, 1]), (4, 8, [1, 2, 1, 1]),
                                       (5, 5, [1, 1, 1, 2, 1]), (2, 6, [2, 1]), (3, 20, [1, 1, 1]),
                                       (3, 16, [2, 1, 1]), (3, 30, [1, 1, 1]), (3, 32, [1, 2, 1]),
-                                      (3, 17, [1, 1, 1])])
+                                      (3, 17, [1, 1, 1]), (3, 9, [1, 1, 2])])
 def test_hot_path_delta_parity(prec, tol, N, R, part):
     """The deltas the mode update's own delta + Gram kernel forms (read back
     through a test hook that stores each tile before its Gram), against the
@@ -273,7 +273,8 @@ def test_error_paths():
                                       (2, 13, [2, 1]), (4, 7, [1, 2, 1, 1]), (4, 9, [1, 1, 2, 1]),
                                       (5, 4, [1, 1, 1, 2, 1]), (5, 5, [1, 1, 1, 2, 1]),
                                       (4, 8, [1, 1, 2, 1]), (4, 4, [2, 1, 1, 1]),
-                                      (3, 20, [2, 1, 2]), (3, 32, [1, 1, 2]), (3, 24, [1, 2, 1])])
+                                      (3, 20, [2, 1, 2]), (3, 32, [1, 1, 2]), (3, 24, [1, 2, 1]),
+                                      (3, 9, [1, 2, 1])])
 def test_mfma_delta_gram_parity(prec, tol_gram, N, R, part):
     """BF16 MFMA delta (plain and with the split-BF16 refinement; above
     R = 10 the refinement takes a second MFMA per tile; orders 4 and 5 run
