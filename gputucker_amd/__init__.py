@@ -60,6 +60,7 @@ class GtPartitionInfo(C.Structure):
         ("stream_bytes_per_nnz", C.c_uint32), ("partition_path", C.c_int),
         ("partition_pieces", C.c_uint64),
         ("host_pageable_bytes", C.c_uint64),
+        ("hot_rows", C.c_uint32 * MAX_ORDER),
     ]
 
 
@@ -265,7 +266,8 @@ class Engine:
                     out_of_core=i.out_of_core, segment_nnz=i.segment_nnz,
                     stream_bytes_per_nnz=i.stream_bytes_per_nnz, partition_path=i.partition_path,
                     partition_pieces=i.partition_pieces,
-                    host_pageable_bytes=i.host_pageable_bytes)
+                    host_pageable_bytes=i.host_pageable_bytes,
+                    hot_rows=np.array(i.hot_rows[:N]))
 
     def get_block(self, b, n):
         idx = np.zeros((self.order, max(n, 1)), np.uint32)

@@ -76,6 +76,10 @@ template <typename T> struct DevBuf {
     p = nullptr;
     n = 0;
   }
+  void swap(DevBuf &other) {
+    std::swap(p, other.p);
+    std::swap(n, other.n);
+  }
   void alloc(size_t count) {
     if (count <= n && p) return;
     reset();
@@ -303,6 +307,15 @@ struct gt_ctx {
   DevBuf<uint64_t> row_ptr[GT_MAX_ORDER];
   DevBuf<uint32_t> where[GT_MAX_ORDER];
   uint64_t rows[GT_MAX_ORDER] = {};
+  // hot outer rows of mode n's wide refined BF16 delta (in-core partitions):
+  // the rows of the outer factor holding so many nonzeros that mode n's rows
+  // carry long runs of them (each row's nonzeros are grouped by them in the
+  // stream); their slot per outer row, the rows, and the contracted-core
+  // tables rebuilt at each mode update
+  int hot_k[GT_MAX_ORDER] = {};
+  DevBuf<uint32_t> hot_slot[GT_MAX_ORDER];
+  DevBuf<uint32_t> hot_rows[GT_MAX_ORDER];
+  DevBuf<uint16_t> hot_tab[GT_MAX_ORDER];
   // model
   bool initialized = false;
   DevBuf<float> A[GT_MAX_ORDER];
@@ -867,6 +880,10 @@ GramArgs gram_args(gt_ctx *c, int n) {
   a.A_L = c->A[L].p;
   a.As_L = c->As[L].p; // null unless a BF16 precision is selected
   a.delta_out = c->dbg_delta;
+  if (c->hot_k[n] > 0 && wide_bf16(c)) {
+    a.hot_slot = c->hot_slot[n].p;
+    a.hot_tab = c->hot_tab[n].p;
+  }
   int t = 0;
   for (int k = 0; k < c->N; ++k) {
     if (k == n || k == L) continue;
@@ -952,6 +969,9 @@ std::pair<uint64_t, uint64_t> stage_segment(gt_ctx *c, int n, uint64_t k) {
 
 void run_gram(gt_ctx *c, int n) {
   GramArgs a = gram_args(c, n);
+  if (a.hot_slot != nullptr) // the outer factor changed since the last mode update
+    launch_hot_tables(c->R, a.A_O[0], c->hot_rows[n].p, c->hot_k[n], a.Gt, c->hot_tab[n].p,
+                      c->s);
   if (!c->streaming) {
     launch_gram_window(c, a);
   } else {
@@ -1324,6 +1344,92 @@ void build_where(gt_ctx *c, int n) {
   GT_HIP(hipStreamSynchronize(s));
 }
 
+// Hot outer rows (wide refined BF16 ranks without a packed tile, order 3).
+// For mode n with outer mode O, a row h of O is hot when it holds at least
+// kHotRunNnz nonzeros per row of mode n: then mode n's rows carry runs of
+// that many nonzeros with outer row h on average, and a run covers whole
+// 16-nonzero MFMA blocks.  The kMaxHotRows heaviest such rows (ties: lower
+// row) get slots, and each row's nonzeros in mode n's stream are grouped by
+// slot (cold ones last), stably, so inside a group the order stays that of
+// the reference's block loop.  The rows, their pointers and the Gram are
+// unchanged; only the order of terms inside a row's sum moves.  Runs after
+// the input copy is freed, with the partition's sort buffers.
+constexpr uint64_t kHotRunNnz = 32;
+constexpr int kHotShift = 6; // slot bits in the sort key: slots 0..62, 63 cold
+
+void clear_hot_rows(gt_ctx *c) {
+  for (int n = 0; n < GT_MAX_ORDER; ++n) {
+    c->hot_k[n] = 0;
+    c->hot_slot[n].reset();
+    c->hot_rows[n].reset();
+    c->hot_tab[n].reset();
+  }
+}
+
+void group_hot_rows(gt_ctx *c, DevBuf<uint32_t> &keys, DevBuf<uint32_t> &vals,
+                    DevBuf<uint32_t> &tk, DevBuf<uint32_t> &tv, DevBuf<char> &temp) {
+  const int N = c->N;
+  clear_hot_rows(c);
+  if (c->o.precision != GT_PREC_BF16R || !wide_hot_supported(c->R, N) || c->nnz == 0) return;
+  static_assert((1 << kHotShift) - 1 >= kMaxHotRows, "slot bits");
+  hipStream_t s = c->s;
+  const uint64_t nnz = c->nnz;
+  DevBuf<uint32_t> tmp;
+  for (int n = 0; n < N; ++n) {
+    const int L = (n == N - 1) ? N - 2 : N - 1;
+    int O = 0;
+    while (O == n || O == L) ++O;
+    const uint64_t rows = c->rows[n], rows_o = c->rows[O];
+    if (ceil_log2(rows) + kHotShift > 32) continue;
+    std::vector<uint64_t> rp(rows_o + 1);
+    GT_HIP(hipMemcpyAsync(rp.data(), c->row_ptr[O].p, rp.size() * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, s));
+    GT_HIP(hipStreamSynchronize(s));
+    std::vector<std::pair<uint64_t, uint32_t>> cand; // (degree, row)
+    for (uint64_t h = 0; h < rows_o; ++h) {
+      const uint64_t d = rp[h + 1] - rp[h];
+      if (d >= kHotRunNnz * rows) cand.emplace_back(d, (uint32_t)h);
+    }
+    if (cand.empty()) continue;
+    std::stable_sort(cand.begin(), cand.end(),
+                     [](const auto &x, const auto &y) { return x.first > y.first; });
+    const int K = (int)std::min<size_t>(cand.size(), kMaxHotRows);
+    std::vector<uint32_t> slot(rows_o, kNoHotSlot), hrows(K);
+    for (int k = 0; k < K; ++k) {
+      hrows[k] = cand[k].second;
+      slot[cand[k].second] = (uint32_t)k;
+    }
+    c->hot_slot[n].alloc(rows_o);
+    c->hot_rows[n].alloc(K);
+    c->hot_tab[n].alloc(hot_table_words(c->R, K));
+    GT_HIP(hipMemcpyAsync(c->hot_slot[n].p, slot.data(), rows_o * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+    GT_HIP(hipMemcpyAsync(c->hot_rows[n].p, hrows.data(), K * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+    // stable sort of the stream by (row, slot)
+    hot_keys(c->st_idx[n][n].p, c->st_idx[n][O].p, c->hot_slot[n].p, kHotShift, nnz, keys.p, s);
+    iota_u32(vals.p, nnz, s);
+    radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(rows) + kHotShift, temp.p, s);
+    tmp.alloc(nnz);
+    for (int k = 0; k < N; ++k) {
+      U32Ptrs src{};
+      U32MutPtrs dst{};
+      src.p[0] = c->st_idx[n][k].p;
+      dst.p[0] = tmp.p;
+      gather_coo(1, nnz, vals.p, src, nullptr, dst, nullptr, s);
+      c->st_idx[n][k].swap(tmp);
+    }
+    DevBuf<float> tmpv;
+    GT_HIP(hipStreamSynchronize(s));
+    tmp.reset(); // the value copy takes its place
+    tmpv.alloc(nnz);
+    gather_coo(0, nnz, vals.p, U32Ptrs{}, c->st_val[n].p, U32MutPtrs{}, tmpv.p, s);
+    c->st_val[n].swap(tmpv);
+    GT_HIP(hipStreamSynchronize(s)); // before the swapped-out copies are freed
+    c->hot_k[n] = K;
+  }
+}
+
 void do_partition_host(gt_ctx *c);
 
 void do_partition(gt_ctx *c) {
@@ -1435,6 +1541,7 @@ void do_partition(gt_ctx *c) {
   // the raw input copy is no longer needed
   for (int k = 0; k < N; ++k) c->in_idx[k].reset();
   c->in_val.reset();
+  group_hot_rows(c, keys, vals, tk, tv, temp);
   c->have_coo = false;
   c->partitioned = true;
   ensure_work(c);
@@ -1471,6 +1578,7 @@ void do_partition_host(gt_ctx *c) {
   };
   c->host_partitioned = true;
   c->partition_pieces = 0;
+  clear_hot_rows(c); // the bucketed partition keeps each row in block order
   Plan p;
   if (!plan_partition(N, c->dims, nnz, c->Ru, budget, (int)sizeof(float), c->o.partition, p))
     throw GtError(GT_ERR_PARTITION, "Block dimension is larger than the tensor dimension.");
@@ -2013,6 +2121,7 @@ int gt_get_partition_info(gt_ctx *ctx, gt_partition_info *info) {
     info->partition_path = ctx->host_partitioned ? 1 : 0;
     info->partition_pieces = ctx->partition_pieces;
     info->host_pageable_bytes = ctx->pageable_bytes;
+    for (int n = 0; n < GT_MAX_ORDER; ++n) info->hot_rows[n] = (uint32_t)ctx->hot_k[n];
   });
 }
 

@@ -76,6 +76,24 @@ k_gram_bf16w(GramArgs a, int wpb) {
     const uint32_t iL = a.idx_L[ee];
     const uint32_t io = a.idx_O[0][ee];
     const float x = valid ? a.st.val[ee] : 0.f;
+    // A block of 16 nonzeros that share one hot outer row h has the delta
+    // sum_jL A_L[jL] M_h[jL][r], M_h the core contracted with that row: one
+    // refined MFMA set per output tile instead of one per outer index.
+    // Bit mb of hot4 (wave-uniform): block mb is such a block.
+    uint32_t hot4 = 0;
+    uint32_t hslot[4] = {0u, 0u, 0u, 0u};
+    if constexpr (PO == 1) {
+      if (a.hot_slot != nullptr) { // a kernel argument: uniform
+        const uint32_t sl = valid ? a.hot_slot[io] : kNoHotSlot;
+        const uint32_t io_first = (uint32_t)__shfl((int)io, lane & 48);
+        const uint64_t same = __ballot(sl != kNoHotSlot && io == io_first);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+          if (((same >> (16 * mb)) & 0xFFFFull) == 0xFFFFull) hot4 |= 1u << mb;
+          hslot[mb] = __builtin_amdgcn_readlane(sl, 16 * mb);
+        }
+      }
+    }
     // lane (q, g) takes, for block mb, the split words of nonzero 16 mb + q:
     // hi and lo of columns 8g..8g+7 of its A_L row
     u4 ah[4], al[4];
@@ -86,13 +104,37 @@ k_gram_bf16w(GramArgs a, int wpb) {
       ah[mb] = p[0];
       al[mb] = p[1];
     }
-    {
+    if (hot4 != 0xFu) { // the outer rows' s values, unless every block is hot
       float ro[RS];
       load_row<R>(a.A_O[0], io, ro);
 #pragma unroll
       for (int o = 0; o < R; ++o) buf[o * kWave + lane] = ro[o];
     }
     s_row[lane] = valid ? row : 0xFFFFFFFFu;
+    f4 dacc[4][MT]; // [mb][MT - 1]: the packed last tile, when PO > 1
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) dacc[mb][mt] = f4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (PO == 1) {
+      if (hot4 != 0) { // the hot blocks: U = A_L M_h is the delta itself
+        const u4 *htab = reinterpret_cast<const u4 *>(a.hot_tab);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+          if (!((hot4 >> mb) & 1u)) continue;
+          const b8 bh = __builtin_bit_cast(b8, ah[mb]), bl = __builtin_bit_cast(b8, al[mb]);
+#pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const size_t at = (((size_t)hslot[mb] * MT + mt) * kWave + lane) * 2;
+            const b8 th = __builtin_bit_cast(b8, htab[at]), tl = __builtin_bit_cast(b8, htab[at + 1]);
+            f4 u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(th, bh, (f4{0.f, 0.f, 0.f, 0.f}), 0, 0, 0);
+            u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(th, bl, u, 0, 0, 0);
+            u = __builtin_amdgcn_mfma_f32_16x16x32_bf16(tl, bh, u, 0, 0, 0);
+            dacc[mb][mt] = u;
+          }
+        }
+      }
+    }
     u4 nh[MTO], nl[MTO];
 #pragma unroll
     for (int mt = 0; mt < MTO; ++mt) {
@@ -100,14 +142,10 @@ k_gram_bf16w(GramArgs a, int wpb) {
       nl[mt] = tab[((size_t)mt * kWave + lane) * 2 + 1];
     }
     wave_sync();
-    f4 dacc[4][MT]; // [mb][MT - 1]: the packed last tile, when PO > 1
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) dacc[mb][mt] = f4{0.f, 0.f, 0.f, 0.f};
     u4 ph = u4{0u, 0u, 0u, 0u}, pl = ph; // the packed tile's core operand of this block
+    const int o_end = hot4 == 0xFu ? 0 : R;
 #pragma unroll 1
-    for (int o = 0; o < R; ++o) {
+    for (int o = 0; o < o_end; ++o) {
       u4 gh[MTO], gl[MTO];
 #pragma unroll
       for (int mt = 0; mt < MTO; ++mt) {
@@ -131,6 +169,7 @@ k_gram_bf16w(GramArgs a, int wpb) {
       }
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
+        if ((hot4 >> mb) & 1u) continue; // (uniform)
         const float sv = buf[o * kWave + mb * 16 + q];
         const b8 bh = __builtin_bit_cast(b8, ah[mb]), bl = __builtin_bit_cast(b8, al[mb]);
 #pragma unroll
@@ -222,6 +261,41 @@ __global__ void __launch_bounds__(256) k_split_wide(const float *__restrict__ A,
   dst[1] = lo;
 }
 
+// The hot-row tables: for slot k (outer row h = hot_rows[k]) the core
+// contracted with that row, M[jL][r] = sum_o A_O[h][o] G[o][jL][r] (FP64 sums
+// of FP32 products, rounded to FP32), as split BF16 operand words in the
+// core table's layout for one outer index: [k][mt][lane][hi 8, lo 8], lane
+// (q, g) holding M[8g..8g+7][16mt + q], zero at and beyond R.  One thread per
+// two words (a pair of jL).
+template <int R>
+__global__ void __launch_bounds__(256) k_hot_tables(const float *__restrict__ A_O,
+                                                    const uint32_t *__restrict__ hot_rows, int K,
+                                                    const float *__restrict__ Gt,
+                                                    uint32_t *__restrict__ out) {
+  constexpr int MT = wide_tiles(R), RS = row_stride(R);
+  constexpr uint64_t per_slot = (uint64_t)MT * kWave * 4;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)K * per_slot) return;
+  const uint64_t slot = t / per_slot;
+  const int rem = (int)(t % per_slot);
+  const int k = rem & 3, lane = (rem >> 2) & (kWave - 1), mt = rem >> 8;
+  const int r = 16 * mt + (lane & 15), j0 = 8 * (lane >> 4) + 2 * k;
+  const float *arow = A_O + (size_t)hot_rows[slot] * RS;
+  float v[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int jL = j0 + i;
+    double sum = 0.0;
+    if (jL < R && r < R)
+      for (int o = 0; o < R; ++o) sum += (double)arow[o] * (double)Gt[((size_t)o * R + jL) * R + r];
+    v[i] = (float)sum;
+  }
+  const uint32_t h = pk_bf16(v[0], v[1]);
+  uint32_t *dst = out + ((slot * MT + mt) * kWave + lane) * 8;
+  dst[k] = h;
+  dst[4 + k] = pk_bf16(v[0] - bf16_low_f(h), v[1] - bf16_high_f(h));
+}
+
 template <int R> void wide_impl(const GramArgs &a, hipStream_t s) {
   const size_t per_wave = (size_t)WideLayout<R>::PER_WAVE * sizeof(float);
   const int wpb = 4;
@@ -246,6 +320,30 @@ void launch_gram_bf16_wide(int R, const GramArgs &a, hipStream_t s) {
 #undef GT_CASE
   default:
     throw std::invalid_argument("the wide BF16 delta is built for R = 16, 20, 24, 28, 30, 32");
+  }
+  GT_HIP(hipGetLastError());
+}
+
+bool wide_hot_supported(int R, int order) {
+  return gram_wide_supported(R, order) && wide_pack(R) == 1;
+}
+
+uint64_t hot_table_words(int R, int K) { return (uint64_t)K * wide_tiles(R) * kWave * 16; }
+
+void launch_hot_tables(int R, const float *A_O, const uint32_t *hot_rows, int K, const float *Gt,
+                       uint16_t *out, hipStream_t s) {
+  if (K <= 0) return;
+  const unsigned grid = (unsigned)(((uint64_t)K * wide_tiles(R) * kWave * 4 + 255) / 256);
+  uint32_t *o32 = reinterpret_cast<uint32_t *>(out);
+  switch (R) {
+#define GT_CASE(r)                                                                               \
+  case r:                                                                                        \
+    hipLaunchKernelGGL(k_hot_tables<r>, dim3(grid), dim3(256), 0, s, A_O, hot_rows, K, Gt, o32); \
+    break;
+    GT_CASE(16) GT_CASE(28) GT_CASE(30) GT_CASE(32)
+#undef GT_CASE
+  default:
+    throw std::invalid_argument("the hot-row tables are built for R = 16, 28, 30, 32");
   }
   GT_HIP(hipGetLastError());
 }

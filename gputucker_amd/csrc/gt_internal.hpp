@@ -131,6 +131,11 @@ int sum_squares(const float *x, uint64_t n, double *out, hipStream_t s);
 // keys[e] = block id of block-ordered position perm[e] (via blk[]), vals[e] = perm[e].
 void keys_from_perm(const uint32_t *perm, const uint32_t *blk, uint64_t n, uint32_t *keys,
                     hipStream_t s);
+// keys[e] = row[e] << shift | slot, slot = slot_of[outer[e]] capped at
+// 2^shift - 1 (the cold rows): sorting by these keys groups each row's
+// nonzeros by the hot outer row they share.
+void hot_keys(const uint32_t *row, const uint32_t *outer, const uint32_t *slot_of, int shift,
+              uint64_t n, uint32_t *keys, hipStream_t s);
 // where[j] = pos[j] - block_off[blk[pos[j]]]
 void local_positions(const uint32_t *pos, const uint32_t *blk, const uint64_t *block_off,
                      uint64_t n, uint32_t *where, hipStream_t s);
@@ -185,6 +190,12 @@ struct GramArgs {
   int alt_source;                        // 1: refined BF16 takes its core operand from the other
                                          // source than the default (LDS table or L2), where
                                          // both are built (the autotuner's choice)
+  // Hot outer rows (wide refined BF16, order 3): slot of each row of the
+  // outer factor (~0u: none), and per slot the core contracted with that
+  // row, M[jL][r] = sum_o A_O[h][o] G[o][jL][r], as split BF16 operand words
+  // in the core table's lane order.  Null: no hot rows.
+  const uint32_t *hot_slot;
+  const void *hot_tab;
 };
 void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s);
 void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);
@@ -296,6 +307,15 @@ constexpr int kSplitWordsWide = 32;
 bool gram_wide_supported(int R, int order);
 void launch_gram_bf16_wide(int R, const GramArgs &a, hipStream_t s);
 void launch_split_wide(int R, const float *A, uint64_t rows, uint32_t *As, hipStream_t s);
+// Hot outer rows of the wide kernel (ranks whose last tile is not packed:
+// R = 16, 28, 30, 32): GramArgs.hot_slot / hot_tab.  A slot table entry of
+// kNoHotSlot marks a cold row; at most kMaxHotRows slots.
+constexpr uint32_t kNoHotSlot = 0xFFFFFFFFu;
+constexpr int kMaxHotRows = 63;
+bool wide_hot_supported(int R, int order);
+uint64_t hot_table_words(int R, int K); // uint16 words of K slots' tables
+void launch_hot_tables(int R, const float *A_O, const uint32_t *hot_rows, int K, const float *Gt,
+                       uint16_t *out, hipStream_t s);
 // The last 16-column output tile of the wide kernel holds RR = R - 16 (MT - 1)
 // columns; when RR <= 8 it packs wide_pack(R) outer indices (rows (o - PO ob,
 // r) with r padded to 16 / PO), one MFMA set per PO outer indices.

@@ -151,6 +151,17 @@ __global__ void k_keys_from_perm(const uint32_t *__restrict__ perm, const uint32
     keys[e] = blk[perm[e]];
 }
 
+__global__ void k_hot_keys(const uint32_t *__restrict__ row, const uint32_t *__restrict__ outer,
+                           const uint32_t *__restrict__ slot_of, int shift, uint64_t n,
+                           uint32_t *__restrict__ keys) {
+  const uint32_t cold = (1u << shift) - 1;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t sl = slot_of[outer[e]];
+    keys[e] = (row[e] << shift) | (sl < cold ? sl : cold);
+  }
+}
+
 __global__ void k_local_positions(const uint32_t *__restrict__ pos, const uint32_t *__restrict__ blk,
                                   const uint64_t *__restrict__ block_off, uint64_t n,
                                   uint32_t *__restrict__ where) {
@@ -312,6 +323,14 @@ void keys_from_perm(const uint32_t *perm, const uint32_t *blk, uint64_t n, uint3
   if (!n) return;
   hipLaunchKernelGGL(k_keys_from_perm, dim3(grid_for(n, 256)), dim3(256), 0, s, perm, blk, n,
                      keys);
+  GT_HIP(hipGetLastError());
+}
+
+void hot_keys(const uint32_t *row, const uint32_t *outer, const uint32_t *slot_of, int shift,
+              uint64_t n, uint32_t *keys, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_hot_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, row, outer, slot_of,
+                     shift, n, keys);
   GT_HIP(hipGetLastError());
 }
 

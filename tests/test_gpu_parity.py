@@ -673,6 +673,67 @@ def test_core_source_variants(N, R, part, paths):
         e.close()
 
 
+@pytest.mark.parametrize("R", [16, 30, 32])
+def test_wide_hot_outer_rows(R):
+    """Order 3 with mode 0 Zipf(1.5) over 40 rows: a few rows of mode 0 hold
+    at least 32 nonzeros per row of modes 1 and 2, so those modes' refined
+    BF16 wide delta has hot outer rows.  Each row's nonzeros are grouped in
+    the stream by hot outer row (slots by degree, heaviest first, the cold
+    ones last, stably), which a numpy stable sort of the input order pins
+    (P = 1: block order is input order).  A 16-nonzero block sharing one hot
+    row takes the core contracted with it, M[jL][r] = sum_o A_0[h][o]
+    G[o][jL][r]: its deltas (delta.cuh:62-90), B and C (update.cuh:35-101)
+    against the FP64 oracle at the refined BF16 bars, and the fit after one
+    iteration within 1e-4 of the FP64 oracle's.  The FP32 path keeps the
+    plain stream."""
+    from gputucker_amd import synth
+    print(f"wide hot outer rows: R={R}", flush=True)
+    idx, vals, dims = synth.generate([40, 250, 150], 40000, [("zipf", 1.5), "uniform", "uniform"],
+                                     "uniform", seed=R)
+    part = [1, 1, 1]
+    A, G = random_init(R, dims, R)
+    e = make(idx, vals, dims, R, part, precision="bf16r", chunk_tiles=1)
+    e.set_init(A, G)
+    hot = e.info()["hot_rows"]
+    t = oracle.Tensor(idx, vals.astype(np.float64), dims, part, R)
+    Ap = [oracle.pad_factor(a.astype(np.float64), t.rows(k)) for k, a in enumerate(A)]
+    Gd = G.astype(np.float64)
+    lam = gt.default_lambda()
+    scale_n = max(1, R ** 2 / 100)
+    outer = {0: 1, 1: 0, 2: 0}
+    for m in range(3):
+        O = outer[m]
+        deg = np.bincount(idx[O], minlength=dims[O])
+        cand = [h for h in np.lexsort((np.arange(dims[O]), -deg)) if deg[h] >= 32 * dims[m]][:63]
+        assert hot[m] == len(cand), (m, hot, len(cand))
+        slot = np.full(dims[O], 63, np.int64)
+        slot[cand] = np.arange(len(cand))
+        perm = np.argsort(idx[m].astype(np.int64) * 64 + slot[idx[O]], kind="stable")
+        d, idx_s = e.get_stream_delta(m)
+        assert np.array_equal(idx_s, idx[:, perm]), m
+        ref = oracle.delta(t, Ap, Gd, m)
+        dr = ref[_stream_positions(t, idx_s)]
+        sc = np.abs(dr).max(axis=1, keepdims=True) + 1e-30
+        assert (np.abs(d - dr) / sc).max() < 1e-4 * scale_n, m
+        B, Cv = oracle.bc(t, ref, m, lam)
+        Bg, Cg = e.get_gram(m, t.rows(m))
+        sb = np.abs(B).max(axis=(1, 2), keepdims=True)
+        assert (np.abs(Bg - B) / sb).max() < 2e-4, m
+        scc = np.abs(Cv).max(axis=1, keepdims=True) + 1e-30
+        assert (np.abs(Cg - Cv) / scc).max() < 2e-4, m
+    assert hot[0] == 0 and hot[1] >= 1 and hot[2] >= 1, hot
+    e.close()
+    f = make(idx, vals, dims, R, part, precision="fp32", chunk_tiles=1)
+    assert f.info()["hot_rows"].tolist() == [0, 0, 0]
+    f.close()
+    e = make(idx, vals, dims, R, part, precision="bf16r", max_iters=1, tol=0.0)
+    e.set_init(A, G)
+    fits = e.run()
+    f64, _ = oracle.als(t, Ap, Gd, lam, 1, 0.0, "f64")
+    assert abs(fits[0] - f64[0]) / abs(f64[0]) < 1e-4, (fits, f64)
+    e.close()
+
+
 def test_host_partition_windows_inside_a_heavy_block():
     """A row holding most of the tensor in one block (P = 1): under a tight
     device budget the bucketed partition cuts it into windows of its input
