@@ -1347,14 +1347,15 @@ void build_where(gt_ctx *c, int n) {
 // Hot outer rows (wide refined BF16 ranks without a packed tile, order 3).
 // For mode n with outer mode O, a row h of O is hot when it holds at least
 // kHotRunNnz nonzeros per row of mode n: then mode n's rows carry runs of
-// that many nonzeros with outer row h on average, and a run covers whole
-// 16-nonzero MFMA blocks.  The kMaxHotRows heaviest such rows (ties: lower
+// that many nonzeros with outer row h on average, long enough to cover whole
+// 16-nonzero MFMA blocks (on the order-3 Zipf(1.5) workload, 16 beat 32 and
+// 64 by 1-5 %).  The kMaxHotRows heaviest such rows (ties: lower
 // row) get slots, and each row's nonzeros in mode n's stream are grouped by
 // slot (cold ones last), stably, so inside a group the order stays that of
 // the reference's block loop.  The rows, their pointers and the Gram are
 // unchanged; only the order of terms inside a row's sum moves.  Runs after
 // the input copy is freed, with the partition's sort buffers.
-constexpr uint64_t kHotRunNnz = 32;
+constexpr uint64_t kHotRunNnz = 16;
 constexpr int kHotShift = 6; // slot bits in the sort key: slots 0..62, 63 cold
 
 void clear_hot_rows(gt_ctx *c) {

@@ -42,10 +42,13 @@ template <int R> struct WideLayout {
   static constexpr int PER_WAVE = BUF + kWave;       // + the tile's row ids
 };
 
-template <int R>
+// HOT: the instance for a mode with hot outer rows (a.hot_slot set); the
+// other one carries none of their code.
+template <int R, bool HOT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_gram_bf16w(GramArgs a, int wpb) {
   static_assert(R >= 16 && R <= 32, "the wide kernel covers 16 <= R <= 32");
+  static_assert(!HOT || wide_pack(R) == 1, "hot outer rows need an unpacked last tile");
   using Lay = WideLayout<R>;
   constexpr int MT = Lay::MT, MTO = Lay::MTO, PO = Lay::PO, RP = Lay::RP, RS = row_stride(R);
   constexpr int RPr = 16 / PO; // rows per outer index in the packed last tile
@@ -82,16 +85,14 @@ k_gram_bf16w(GramArgs a, int wpb) {
     // Bit mb of hot4 (wave-uniform): block mb is such a block.
     uint32_t hot4 = 0;
     uint32_t hslot[4] = {0u, 0u, 0u, 0u};
-    if constexpr (PO == 1) {
-      if (a.hot_slot != nullptr) { // a kernel argument: uniform
-        const uint32_t sl = valid ? a.hot_slot[io] : kNoHotSlot;
-        const uint32_t io_first = (uint32_t)__shfl((int)io, lane & 48);
-        const uint64_t same = __ballot(sl != kNoHotSlot && io == io_first);
+    if constexpr (HOT) {
+      const uint32_t sl = valid ? a.hot_slot[io] : kNoHotSlot;
+      const uint32_t io_first = (uint32_t)__shfl((int)io, lane & 48);
+      const uint64_t same = __ballot(sl != kNoHotSlot && io == io_first);
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-          if (((same >> (16 * mb)) & 0xFFFFull) == 0xFFFFull) hot4 |= 1u << mb;
-          hslot[mb] = __builtin_amdgcn_readlane(sl, 16 * mb);
-        }
+      for (int mb = 0; mb < 4; ++mb) {
+        if (((same >> (16 * mb)) & 0xFFFFull) == 0xFFFFull) hot4 |= 1u << mb;
+        hslot[mb] = __builtin_amdgcn_readlane(sl, 16 * mb);
       }
     }
     // lane (q, g) takes, for block mb, the split words of nonzero 16 mb + q:
@@ -116,7 +117,7 @@ k_gram_bf16w(GramArgs a, int wpb) {
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) dacc[mb][mt] = f4{0.f, 0.f, 0.f, 0.f};
-    if constexpr (PO == 1) {
+    if constexpr (HOT) {
       if (hot4 != 0) { // the hot blocks: U = A_L M_h is the delta itself
         const u4 *htab = reinterpret_cast<const u4 *>(a.hot_tab);
 #pragma unroll
@@ -303,7 +304,15 @@ template <int R> void wide_impl(const GramArgs &a, hipStream_t s) {
     throw std::invalid_argument("wide BF16 Gram: LDS per workgroup exceeds the device's");
   if (a.chunk_end <= a.chunk_begin) return;
   const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
-  hipLaunchKernelGGL(k_gram_bf16w<R>, dim3(grid), dim3(64 * wpb), per_wave * wpb, s, a, wpb);
+  if constexpr (wide_pack(R) == 1) {
+    if (a.hot_slot != nullptr) {
+      hipLaunchKernelGGL((k_gram_bf16w<R, true>), dim3(grid), dim3(64 * wpb), per_wave * wpb, s, a,
+                         wpb);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_gram_bf16w<R, false>), dim3(grid), dim3(64 * wpb), per_wave * wpb, s, a,
+                     wpb);
 }
 
 } // namespace
