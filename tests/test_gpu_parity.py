@@ -676,7 +676,7 @@ def test_core_source_variants(N, R, part, paths):
 @pytest.mark.parametrize("R", [16, 30, 32])
 def test_wide_hot_outer_rows(R):
     """Order 3 with mode 0 Zipf(1.5) over 40 rows: a few rows of mode 0 hold
-    at least 32 nonzeros per row of modes 1 and 2, so those modes' refined
+    at least 16 nonzeros per row of modes 1 and 2, so those modes' refined
     BF16 wide delta has hot outer rows.  Each row's nonzeros are grouped in
     the stream by hot outer row (slots by degree, heaviest first, the cold
     ones last, stably), which a numpy stable sort of the input order pins
@@ -704,7 +704,7 @@ def test_wide_hot_outer_rows(R):
     for m in range(3):
         O = outer[m]
         deg = np.bincount(idx[O], minlength=dims[O])
-        cand = [h for h in np.lexsort((np.arange(dims[O]), -deg)) if deg[h] >= 32 * dims[m]][:63]
+        cand = [h for h in np.lexsort((np.arange(dims[O]), -deg)) if deg[h] >= 16 * dims[m]][:63]
         assert hot[m] == len(cand), (m, hot, len(cand))
         slot = np.full(dims[O], 63, np.int64)
         slot[cand] = np.arange(len(cand))
@@ -721,7 +721,7 @@ def test_wide_hot_outer_rows(R):
         assert (np.abs(Bg - B) / sb).max() < 2e-4, m
         scc = np.abs(Cv).max(axis=1, keepdims=True) + 1e-30
         assert (np.abs(Cg - Cv) / scc).max() < 2e-4, m
-    assert hot[0] == 0 and hot[1] >= 1 and hot[2] >= 1, hot
+    assert hot[0] == 0 and hot[1] >= 2 and hot[2] >= 2, hot
     e.close()
     f = make(idx, vals, dims, R, part, precision="fp32", chunk_tiles=1)
     assert f.info()["hot_rows"].tolist() == [0, 0, 0]
