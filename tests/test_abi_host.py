@@ -103,3 +103,37 @@ def test_create_rejects_bad_options():
     with pytest.raises(gt.GtError) as e:
         gt.Engine(3, 4, lam=0.0)
     assert e.value.code == 1
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of gt_opts, gt_partition_info and gt_timings have
+    the header's size and field offsets (a C program built from the header
+    prints them): a field added on one side only would make the library
+    write past the caller's struct."""
+    import ctypes
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    structs = {"gt_opts": gt.GtOpts, "gt_partition_info": gt.GtPartitionInfo,
+               "gt_timings": gt.GtTimings}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "gputucker_hip.h"',
+             "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f in py._fields_:  # (a Python keyword gets a trailing underscore: lambda_)
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0].rstrip("_")}));')
+    lines += ["  return 0;", "}"]
+    src, exe = tmp_path / "layout.c", tmp_path / "layout"
+    src.write_text("\n".join(lines) + "\n")
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HDR), str(src), "-o", str(exe)],
+                   check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                               check=True).stdout.splitlines():
+        c, field, v = line.split()
+        got[(c, field)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
