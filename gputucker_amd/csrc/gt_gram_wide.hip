@@ -21,6 +21,11 @@
 // across tiles: the other waves of the SIMD run while one waits.  The tile
 // of deltas then goes through LDS to the FP64 Gram with a block of entries
 // per lane (BlockGram), which reuses the LDS of the s values.
+// Hot outer rows: where a few rows of the outer factor hold most nonzeros,
+// the partition groups each row's nonzeros by them, and a 16-nonzero block
+// sharing one such row h takes U = A_L M_h with M_h = sum_o A_O[h][o] G[o]
+// (formed once per mode update by k_hot_tables): one refined MFMA set per
+// output tile and no stage 2, instead of one per outer index.
 #include "gt_bf16.hpp"
 #include "gt_contract.hpp"
 
