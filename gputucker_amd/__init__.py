@@ -60,7 +60,7 @@ class GtPartitionInfo(C.Structure):
         ("stream_bytes_per_nnz", C.c_uint32), ("partition_path", C.c_int),
         ("partition_pieces", C.c_uint64),
         ("host_pageable_bytes", C.c_uint64),
-        ("hot_rows", C.c_uint32 * MAX_ORDER),
+        ("hot_rows", C.c_uint32 * MAX_ORDER), ("hot_kind", C.c_int * MAX_ORDER),
     ]
 
 
@@ -267,7 +267,7 @@ class Engine:
                     stream_bytes_per_nnz=i.stream_bytes_per_nnz, partition_path=i.partition_path,
                     partition_pieces=i.partition_pieces,
                     host_pageable_bytes=i.host_pageable_bytes,
-                    hot_rows=np.array(i.hot_rows[:N]))
+                    hot_rows=np.array(i.hot_rows[:N]), hot_kind=np.array(i.hot_kind[:N]))
 
     def get_block(self, b, n):
         idx = np.zeros((self.order, max(n, 1)), np.uint32)

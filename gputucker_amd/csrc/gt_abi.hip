@@ -312,7 +312,8 @@ struct gt_ctx {
   // carry long runs of them (each row's nonzeros are grouped by them in the
   // stream); their slot per outer row, the rows, and the contracted-core
   // tables rebuilt at each mode update
-  int hot_k[GT_MAX_ORDER] = {};
+  int hot_kind[GT_MAX_ORDER] = {}; // kHotOuter, kHotLast or 0
+  uint64_t hot_k[GT_MAX_ORDER] = {}; // tables
   DevBuf<uint32_t> hot_slot[GT_MAX_ORDER];
   DevBuf<uint32_t> hot_rows[GT_MAX_ORDER];
   DevBuf<uint16_t> hot_tab[GT_MAX_ORDER];
@@ -880,16 +881,18 @@ GramArgs gram_args(gt_ctx *c, int n) {
   a.A_L = c->A[L].p;
   a.As_L = c->As[L].p; // null unless a BF16 precision is selected
   a.delta_out = c->dbg_delta;
-  if (c->hot_k[n] > 0 && wide_bf16(c)) {
-    a.hot_slot = c->hot_slot[n].p;
-    a.hot_tab = c->hot_tab[n].p;
-  }
   int t = 0;
   for (int k = 0; k < c->N; ++k) {
     if (k == n || k == L) continue;
     a.idx_O[t] = c->st_idx[n][k].p;
     a.A_O[t] = c->A[k].p;
+    if (t == 0) a.As_O = c->As[k].p;
     ++t;
+  }
+  if (c->hot_k[n] > 0 && wide_bf16(c)) {
+    a.hot_kind = c->hot_kind[n];
+    a.hot_slot = c->hot_kind[n] == kHotOuter ? c->hot_slot[n].p : nullptr;
+    a.hot_tab = c->hot_tab[n].p;
   }
   return a;
 }
@@ -969,9 +972,12 @@ std::pair<uint64_t, uint64_t> stage_segment(gt_ctx *c, int n, uint64_t k) {
 
 void run_gram(gt_ctx *c, int n) {
   GramArgs a = gram_args(c, n);
-  if (a.hot_slot != nullptr) // the outer factor changed since the last mode update
-    launch_hot_tables(c->R, a.A_O[0], c->hot_rows[n].p, c->hot_k[n], a.Gt, c->hot_tab[n].p,
+  if (a.hot_tab != nullptr) { // the factors changed since the last mode update
+    const bool outer = a.hot_kind == kHotOuter;
+    launch_hot_tables(c->R, a.hot_kind, outer ? a.A_O[0] : a.A_L,
+                      outer ? c->hot_rows[n].p : nullptr, c->hot_k[n], a.Gt, c->hot_tab[n].p,
                       c->s);
+  }
   if (!c->streaming) {
     launch_gram_window(c, a);
   } else {
@@ -1344,27 +1350,68 @@ void build_where(gt_ctx *c, int n) {
   GT_HIP(hipStreamSynchronize(s));
 }
 
-// Hot outer rows (wide refined BF16 ranks without a packed tile, order 3).
-// For mode n with outer mode O, a row h of O is hot when it holds at least
-// kHotRunNnz nonzeros per row of mode n: then mode n's rows carry runs of
-// that many nonzeros with outer row h on average, long enough to cover whole
-// 16-nonzero MFMA blocks (on the order-3 Zipf(1.5) workload, 16 beat 32 and
-// 64 by 1-5 %).  The kMaxHotRows heaviest such rows (ties: lower
-// row) get slots, and each row's nonzeros in mode n's stream are grouped by
-// slot (cold ones last), stably, so inside a group the order stays that of
-// the reference's block loop.  The rows, their pointers and the Gram are
-// unchanged; only the order of terms inside a row's sum moves.  Runs after
-// the input copy is freed, with the partition's sort buffers.
+// Shared factor rows (wide refined BF16 ranks without a packed tile, order
+// 3).  Mode n, with L the last other mode and O the remaining one, takes one
+// of two groupings of each row's nonzeros in its stream (stable, so inside a
+// group the order stays that of the reference's block loop; the rows, their
+// pointers and the Gram are unchanged, only the order of terms inside a
+// row's sum moves):
+// - by row of A_L (kind kHotLast), when at least half the nonzeros lie in
+//   rows of mode n holding kHotRunNnz or more nonzeros per row of A_L (such
+//   a row carries runs of that many nonzeros per A_L row on average) and A_L
+//   has at most kHotLastMaxRows rows (a table each, 4 KiB at R = 32; not
+//   under a device budget);
+// - else by hot outer row (kind kHotOuter): a row h of O is hot when it
+//   holds at least kHotRunNnz nonzeros per row of mode n; the kMaxHotRows
+//   heaviest (ties: lower row) get slots, cold rows last.
+// Runs long enough cover whole 16-nonzero MFMA blocks (on the order-3
+// Zipf(1.5) workload, 16 beat 32 and 64 by 1-5 %).  Runs after the input
+// copy is freed, with the partition's sort buffers.
 constexpr uint64_t kHotRunNnz = 16;
 constexpr int kHotShift = 6; // slot bits in the sort key: slots 0..62, 63 cold
+constexpr uint64_t kHotLastMaxRows = 1ull << 20;
 
 void clear_hot_rows(gt_ctx *c) {
   for (int n = 0; n < GT_MAX_ORDER; ++n) {
+    c->hot_kind[n] = 0;
     c->hot_k[n] = 0;
     c->hot_slot[n].reset();
     c->hot_rows[n].reset();
     c->hot_tab[n].reset();
   }
+}
+
+// Reorders mode n's stream (every index array and the values) by perm,
+// one array at a time through tmp.
+void permute_stream(gt_ctx *c, int n, const uint32_t *perm, DevBuf<uint32_t> &tmp) {
+  hipStream_t s = c->s;
+  const uint64_t nnz = c->nnz;
+  tmp.alloc(nnz);
+  for (int k = 0; k < c->N; ++k) {
+    U32Ptrs src{};
+    U32MutPtrs dst{};
+    src.p[0] = c->st_idx[n][k].p;
+    dst.p[0] = tmp.p;
+    gather_coo(1, nnz, perm, src, nullptr, dst, nullptr, s);
+    c->st_idx[n][k].swap(tmp);
+  }
+  DevBuf<float> tmpv;
+  GT_HIP(hipStreamSynchronize(s));
+  tmp.reset(); // the value copy takes its place
+  tmpv.alloc(nnz);
+  gather_coo(0, nnz, perm, U32Ptrs{}, c->st_val[n].p, U32MutPtrs{}, tmpv.p, s);
+  c->st_val[n].swap(tmpv);
+  GT_HIP(hipStreamSynchronize(s)); // before the swapped-out copies are freed
+}
+
+std::vector<uint64_t> row_degrees(gt_ctx *c, int m) {
+  std::vector<uint64_t> rp(c->rows[m] + 1);
+  GT_HIP(hipMemcpyAsync(rp.data(), c->row_ptr[m].p, rp.size() * sizeof(uint64_t),
+                        hipMemcpyDeviceToHost, c->s));
+  GT_HIP(hipStreamSynchronize(c->s));
+  for (uint64_t i = 0; i + 1 < rp.size(); ++i) rp[i] = rp[i + 1] - rp[i];
+  rp.pop_back();
+  return rp;
 }
 
 void group_hot_rows(gt_ctx *c, DevBuf<uint32_t> &keys, DevBuf<uint32_t> &vals,
@@ -1380,17 +1427,36 @@ void group_hot_rows(gt_ctx *c, DevBuf<uint32_t> &keys, DevBuf<uint32_t> &vals,
     const int L = (n == N - 1) ? N - 2 : N - 1;
     int O = 0;
     while (O == n || O == L) ++O;
-    const uint64_t rows = c->rows[n], rows_o = c->rows[O];
-    if (ceil_log2(rows) + kHotShift > 32) continue;
-    std::vector<uint64_t> rp(rows_o + 1);
-    GT_HIP(hipMemcpyAsync(rp.data(), c->row_ptr[O].p, rp.size() * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, s));
-    GT_HIP(hipStreamSynchronize(s));
-    std::vector<std::pair<uint64_t, uint32_t>> cand; // (degree, row)
-    for (uint64_t h = 0; h < rows_o; ++h) {
-      const uint64_t d = rp[h + 1] - rp[h];
-      if (d >= kHotRunNnz * rows) cand.emplace_back(d, (uint32_t)h);
+    const uint64_t rows = c->rows[n], rows_o = c->rows[O], rows_l = c->rows[L];
+    if (c->o.device_budget == 0 && rows_l <= kHotLastMaxRows) {
+      uint64_t long_nnz = 0;
+      for (uint64_t d : row_degrees(c, n))
+        if (d >= kHotRunNnz * rows_l) long_nnz += d;
+      if (2 * long_nnz >= nnz) {
+        // stable sort of the stream by (row, A_L row): by the A_L row, then
+        // stably by row
+        GT_HIP(hipMemcpyAsync(keys.p, c->st_idx[n][L].p, nnz * sizeof(uint32_t),
+                              hipMemcpyDeviceToDevice, s));
+        iota_u32(vals.p, nnz, s);
+        radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(rows_l), temp.p, s);
+        U32Ptrs src{};
+        U32MutPtrs dst{};
+        src.p[0] = c->st_idx[n][n].p;
+        dst.p[0] = keys.p;
+        gather_coo(1, nnz, vals.p, src, nullptr, dst, nullptr, s);
+        radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(rows), temp.p, s);
+        permute_stream(c, n, vals.p, tmp);
+        c->hot_tab[n].alloc(hot_table_words(c->R, rows_l));
+        c->hot_kind[n] = kHotLast;
+        c->hot_k[n] = rows_l;
+        continue;
+      }
     }
+    if (ceil_log2(rows) + kHotShift > 32) continue;
+    const std::vector<uint64_t> deg = row_degrees(c, O);
+    std::vector<std::pair<uint64_t, uint32_t>> cand; // (degree, row)
+    for (uint64_t h = 0; h < rows_o; ++h)
+      if (deg[h] >= kHotRunNnz * rows) cand.emplace_back(deg[h], (uint32_t)h);
     if (cand.empty()) continue;
     std::stable_sort(cand.begin(), cand.end(),
                      [](const auto &x, const auto &y) { return x.first > y.first; });
@@ -1411,23 +1477,9 @@ void group_hot_rows(gt_ctx *c, DevBuf<uint32_t> &keys, DevBuf<uint32_t> &vals,
     hot_keys(c->st_idx[n][n].p, c->st_idx[n][O].p, c->hot_slot[n].p, kHotShift, nnz, keys.p, s);
     iota_u32(vals.p, nnz, s);
     radix_sort_pairs(keys.p, vals.p, tk.p, tv.p, nnz, ceil_log2(rows) + kHotShift, temp.p, s);
-    tmp.alloc(nnz);
-    for (int k = 0; k < N; ++k) {
-      U32Ptrs src{};
-      U32MutPtrs dst{};
-      src.p[0] = c->st_idx[n][k].p;
-      dst.p[0] = tmp.p;
-      gather_coo(1, nnz, vals.p, src, nullptr, dst, nullptr, s);
-      c->st_idx[n][k].swap(tmp);
-    }
-    DevBuf<float> tmpv;
-    GT_HIP(hipStreamSynchronize(s));
-    tmp.reset(); // the value copy takes its place
-    tmpv.alloc(nnz);
-    gather_coo(0, nnz, vals.p, U32Ptrs{}, c->st_val[n].p, U32MutPtrs{}, tmpv.p, s);
-    c->st_val[n].swap(tmpv);
-    GT_HIP(hipStreamSynchronize(s)); // before the swapped-out copies are freed
-    c->hot_k[n] = K;
+    permute_stream(c, n, vals.p, tmp);
+    c->hot_kind[n] = kHotOuter;
+    c->hot_k[n] = (uint64_t)K;
   }
 }
 
@@ -2122,7 +2174,10 @@ int gt_get_partition_info(gt_ctx *ctx, gt_partition_info *info) {
     info->partition_path = ctx->host_partitioned ? 1 : 0;
     info->partition_pieces = ctx->partition_pieces;
     info->host_pageable_bytes = ctx->pageable_bytes;
-    for (int n = 0; n < GT_MAX_ORDER; ++n) info->hot_rows[n] = (uint32_t)ctx->hot_k[n];
+    for (int n = 0; n < GT_MAX_ORDER; ++n) {
+      info->hot_rows[n] = (uint32_t)ctx->hot_k[n];
+      info->hot_kind[n] = ctx->hot_kind[n];
+    }
   });
 }
 

@@ -21,11 +21,15 @@
 // across tiles: the other waves of the SIMD run while one waits.  The tile
 // of deltas then goes through LDS to the FP64 Gram with a block of entries
 // per lane (BlockGram), which reuses the LDS of the s values.
-// Hot outer rows: where a few rows of the outer factor hold most nonzeros,
-// the partition groups each row's nonzeros by them, and a 16-nonzero block
-// sharing one such row h takes U = A_L M_h with M_h = sum_o A_O[h][o] G[o]
-// (formed once per mode update by k_hot_tables): one refined MFMA set per
-// output tile and no stage 2, instead of one per outer index.
+// Shared factor rows: where a few rows of the outer factor hold most
+// nonzeros, the partition groups each row's nonzeros by them, and a
+// 16-nonzero block sharing one such row h takes U = A_L M_h with
+// M_h = sum_o A_O[h][o] G[o] (formed once per mode update by k_hot_tables):
+// one refined MFMA set per output tile and no stage 2, instead of one per
+// outer index.  Where the mode's rows are so long that they hold long runs
+// of every row of A_L, the rows are grouped by their A_L row l instead, and
+// a block sharing l takes A_O N_l with N_l[o][r] = sum_jL A_L[l][jL]
+// G[o][jL][r], gathering the outer factor's split rows.
 #include "gt_bf16.hpp"
 #include "gt_contract.hpp"
 
@@ -47,13 +51,13 @@ template <int R> struct WideLayout {
   static constexpr int PER_WAVE = BUF + kWave;       // + the tile's row ids
 };
 
-// HOT: the instance for a mode with hot outer rows (a.hot_slot set); the
-// other one carries none of their code.
-template <int R, bool HOT>
+// HOT: the instance for a mode with shared factor rows (a.hot_kind, 0 for
+// none); the plain instance carries none of their code.
+template <int R, int HOT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_gram_bf16w(GramArgs a, int wpb) {
   static_assert(R >= 16 && R <= 32, "the wide kernel covers 16 <= R <= 32");
-  static_assert(!HOT || wide_pack(R) == 1, "hot outer rows need an unpacked last tile");
+  static_assert(HOT == 0 || wide_pack(R) == 1, "shared rows need an unpacked last tile");
   using Lay = WideLayout<R>;
   constexpr int MT = Lay::MT, MTO = Lay::MTO, PO = Lay::PO, RP = Lay::RP, RS = row_stride(R);
   constexpr int RPr = 16 / PO; // rows per outer index in the packed last tile
@@ -85,15 +89,17 @@ k_gram_bf16w(GramArgs a, int wpb) {
     const uint32_t io = a.idx_O[0][ee];
     const float x = valid ? a.st.val[ee] : 0.f;
     // A block of 16 nonzeros that share one hot outer row h has the delta
-    // sum_jL A_L[jL] M_h[jL][r], M_h the core contracted with that row: one
-    // refined MFMA set per output tile instead of one per outer index.
-    // Bit mb of hot4 (wave-uniform): block mb is such a block.
+    // sum_jL A_L[jL] M_h[jL][r], M_h the core contracted with that row (or,
+    // sharing one row l of A_L, sum_o A_O[o] N_l[o][r]): one refined MFMA
+    // set per output tile instead of one per outer index.  Bit mb of hot4
+    // (wave-uniform): block mb is such a block, with table hslot[mb].
     uint32_t hot4 = 0;
     uint32_t hslot[4] = {0u, 0u, 0u, 0u};
-    if constexpr (HOT) {
-      const uint32_t sl = valid ? a.hot_slot[io] : kNoHotSlot;
-      const uint32_t io_first = (uint32_t)__shfl((int)io, lane & 48);
-      const uint64_t same = __ballot(sl != kNoHotSlot && io == io_first);
+    if constexpr (HOT != 0) {
+      const uint32_t key = HOT == kHotOuter ? io : iL;
+      const uint32_t sl = !valid ? kNoHotSlot : HOT == kHotOuter ? a.hot_slot[io] : iL;
+      const uint32_t key_first = (uint32_t)__shfl((int)key, lane & 48);
+      const uint64_t same = __ballot(sl != kNoHotSlot && key == key_first);
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
         if (((same >> (16 * mb)) & 0xFFFFull) == 0xFFFFull) hot4 |= 1u << mb;
@@ -101,12 +107,21 @@ k_gram_bf16w(GramArgs a, int wpb) {
       }
     }
     // lane (q, g) takes, for block mb, the split words of nonzero 16 mb + q:
-    // hi and lo of columns 8g..8g+7 of its A_L row
+    // hi and lo of columns 8g..8g+7 of its A_L row (its A_O row in a block
+    // that shares its A_L row)
     u4 ah[4], al[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
-      const uint32_t r = (uint32_t)__shfl((int)iL, mb * 16 + q);
-      const u4 *p = reinterpret_cast<const u4 *>(a.As_L + (size_t)r * kSplitWordsWide + 8 * g);
+      uint32_t src = iL;
+      const uint32_t *base = a.As_L;
+      if constexpr (HOT == kHotLast) {
+        if ((hot4 >> mb) & 1u) { // (uniform)
+          src = io;
+          base = a.As_O;
+        }
+      }
+      const uint32_t r = (uint32_t)__shfl((int)src, mb * 16 + q);
+      const u4 *p = reinterpret_cast<const u4 *>(base + (size_t)r * kSplitWordsWide + 8 * g);
       ah[mb] = p[0];
       al[mb] = p[1];
     }
@@ -122,8 +137,8 @@ k_gram_bf16w(GramArgs a, int wpb) {
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) dacc[mb][mt] = f4{0.f, 0.f, 0.f, 0.f};
-    if constexpr (HOT) {
-      if (hot4 != 0) { // the hot blocks: U = A_L M_h is the delta itself
+    if constexpr (HOT != 0) {
+      if (hot4 != 0) { // the hot blocks: U = A_L M_h (A_O N_l) is the delta itself
         const u4 *htab = reinterpret_cast<const u4 *>(a.hot_tab);
 #pragma unroll
         for (int mb = 0; mb < 4; ++mb) {
@@ -267,33 +282,38 @@ __global__ void __launch_bounds__(256) k_split_wide(const float *__restrict__ A,
   dst[1] = lo;
 }
 
-// The hot-row tables: for slot k (outer row h = hot_rows[k]) the core
-// contracted with that row, M[jL][r] = sum_o A_O[h][o] G[o][jL][r] (FP64 sums
-// of FP32 products, rounded to FP32), as split BF16 operand words in the
-// core table's layout for one outer index: [k][mt][lane][hi 8, lo 8], lane
-// (q, g) holding M[8g..8g+7][16mt + q], zero at and beyond R.  One thread per
-// two words (a pair of jL).
-template <int R>
-__global__ void __launch_bounds__(256) k_hot_tables(const float *__restrict__ A_O,
-                                                    const uint32_t *__restrict__ hot_rows, int K,
+// The shared-row tables.  Kind kHotOuter: for slot k (outer row h =
+// rows[k]) the core contracted with that row, M[jL][r] = sum_o A[h][o]
+// G[o][jL][r]; kind kHotLast: for row l of A (= A_L) N[o][r] = sum_jL
+// A[l][jL] G[o][jL][r].  FP64 sums of FP32 products, rounded to FP32, as
+// split BF16 operand words in the core table's layout for one outer index:
+// [k][mt][lane][hi 8, lo 8], lane (q, g) holding T[8g..8g+7][16mt + q], zero
+// at and beyond R.  One thread per two words (a pair of K indices).
+template <int R, int KIND>
+__global__ void __launch_bounds__(256) k_hot_tables(const float *__restrict__ A,
+                                                    const uint32_t *__restrict__ rows, uint64_t K,
                                                     const float *__restrict__ Gt,
                                                     uint32_t *__restrict__ out) {
   constexpr int MT = wide_tiles(R), RS = row_stride(R);
   constexpr uint64_t per_slot = (uint64_t)MT * kWave * 4;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (uint64_t)K * per_slot) return;
+  if (t >= K * per_slot) return;
   const uint64_t slot = t / per_slot;
   const int rem = (int)(t % per_slot);
   const int k = rem & 3, lane = (rem >> 2) & (kWave - 1), mt = rem >> 8;
   const int r = 16 * mt + (lane & 15), j0 = 8 * (lane >> 4) + 2 * k;
-  const float *arow = A_O + (size_t)hot_rows[slot] * RS;
+  const float *arow = A + (size_t)(KIND == kHotOuter ? rows[slot] : slot) * RS;
   float v[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
-    const int jL = j0 + i;
+    const int kk = j0 + i; // jL (kHotOuter) or o (kHotLast)
     double sum = 0.0;
-    if (jL < R && r < R)
-      for (int o = 0; o < R; ++o) sum += (double)arow[o] * (double)Gt[((size_t)o * R + jL) * R + r];
+    if (kk < R && r < R)
+      for (int u = 0; u < R; ++u) {
+        const size_t g = KIND == kHotOuter ? ((size_t)u * R + kk) * R + r
+                                           : ((size_t)kk * R + u) * R + r;
+        sum += (double)arow[u] * (double)Gt[g];
+      }
     v[i] = (float)sum;
   }
   const uint32_t h = pk_bf16(v[0], v[1]);
@@ -310,14 +330,18 @@ template <int R> void wide_impl(const GramArgs &a, hipStream_t s) {
   if (a.chunk_end <= a.chunk_begin) return;
   const unsigned grid = (unsigned)((a.chunk_end - a.chunk_begin + wpb - 1) / wpb);
   if constexpr (wide_pack(R) == 1) {
-    if (a.hot_slot != nullptr) {
-      hipLaunchKernelGGL((k_gram_bf16w<R, true>), dim3(grid), dim3(64 * wpb), per_wave * wpb, s, a,
-                         wpb);
+    if (a.hot_tab != nullptr && a.hot_kind == kHotOuter) {
+      hipLaunchKernelGGL((k_gram_bf16w<R, kHotOuter>), dim3(grid), dim3(64 * wpb), per_wave * wpb,
+                         s, a, wpb);
+      return;
+    }
+    if (a.hot_tab != nullptr && a.hot_kind == kHotLast) {
+      hipLaunchKernelGGL((k_gram_bf16w<R, kHotLast>), dim3(grid), dim3(64 * wpb), per_wave * wpb,
+                         s, a, wpb);
       return;
     }
   }
-  hipLaunchKernelGGL((k_gram_bf16w<R, false>), dim3(grid), dim3(64 * wpb), per_wave * wpb, s, a,
-                     wpb);
+  hipLaunchKernelGGL((k_gram_bf16w<R, 0>), dim3(grid), dim3(64 * wpb), per_wave * wpb, s, a, wpb);
 }
 
 } // namespace
@@ -342,22 +366,29 @@ bool wide_hot_supported(int R, int order) {
   return gram_wide_supported(R, order) && wide_pack(R) == 1;
 }
 
-uint64_t hot_table_words(int R, int K) { return (uint64_t)K * wide_tiles(R) * kWave * 16; }
+uint64_t hot_table_words(int R, uint64_t K) { return K * wide_tiles(R) * kWave * 16; }
 
-void launch_hot_tables(int R, const float *A_O, const uint32_t *hot_rows, int K, const float *Gt,
-                       uint16_t *out, hipStream_t s) {
-  if (K <= 0) return;
-  const unsigned grid = (unsigned)(((uint64_t)K * wide_tiles(R) * kWave * 4 + 255) / 256);
+void launch_hot_tables(int R, int kind, const float *A, const uint32_t *rows, uint64_t K,
+                       const float *Gt, uint16_t *out, hipStream_t s) {
+  if (K == 0) return;
+  const uint64_t blocks = (K * wide_tiles(R) * kWave * 4 + 255) / 256;
+  if (blocks >= (1ull << 31)) throw std::invalid_argument("too many shared-row tables");
+  const unsigned grid = (unsigned)blocks;
   uint32_t *o32 = reinterpret_cast<uint32_t *>(out);
-  switch (R) {
+  switch (R * 4 + kind) {
 #define GT_CASE(r)                                                                               \
-  case r:                                                                                        \
-    hipLaunchKernelGGL(k_hot_tables<r>, dim3(grid), dim3(256), 0, s, A_O, hot_rows, K, Gt, o32); \
+  case r * 4 + kHotOuter:                                                                        \
+    hipLaunchKernelGGL((k_hot_tables<r, kHotOuter>), dim3(grid), dim3(256), 0, s, A, rows, K, Gt, \
+                       o32);                                                                     \
+    break;                                                                                       \
+  case r * 4 + kHotLast:                                                                         \
+    hipLaunchKernelGGL((k_hot_tables<r, kHotLast>), dim3(grid), dim3(256), 0, s, A, rows, K, Gt,  \
+                       o32);                                                                     \
     break;
     GT_CASE(16) GT_CASE(28) GT_CASE(30) GT_CASE(32)
 #undef GT_CASE
   default:
-    throw std::invalid_argument("the hot-row tables are built for R = 16, 28, 30, 32");
+    throw std::invalid_argument("the shared-row tables are built for R = 16, 28, 30, 32");
   }
   GT_HIP(hipGetLastError());
 }

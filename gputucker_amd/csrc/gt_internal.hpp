@@ -190,13 +190,20 @@ struct GramArgs {
   int alt_source;                        // 1: refined BF16 takes its core operand from the other
                                          // source than the default (LDS table or L2), where
                                          // both are built (the autotuner's choice)
-  // Hot outer rows (wide refined BF16, order 3): slot of each row of the
-  // outer factor (~0u: none), and per slot the core contracted with that
-  // row, M[jL][r] = sum_o A_O[h][o] G[o][jL][r], as split BF16 operand words
-  // in the core table's lane order.  Null: no hot rows.
+  // Shared factor rows of the wide refined BF16 delta (order 3), hot_kind:
+  // kHotOuter: hot_slot gives the slot of each row of the outer factor (~0u:
+  //   none) and slot k's table is M[jL][r] = sum_o A_O[h_k][o] G[o][jL][r];
+  // kHotLast: every row l of A_L has a table, N[o][r] = sum_jL A_L[l][jL]
+  //   G[o][jL][r], and the blocks that share l gather As_O (the split copy
+  //   of the outer factor) instead of As_L.
+  // Tables are split BF16 operand words in the core table's lane order.
+  // hot_tab null: neither.
+  int hot_kind;
   const uint32_t *hot_slot;
   const void *hot_tab;
+  const uint32_t *As_O;
 };
+constexpr int kHotOuter = 1, kHotLast = 2;
 void launch_gram_fp32(int R, const GramArgs &a, hipStream_t s);
 void launch_gram_fixup(int R, const GramArgs &a, hipStream_t s);
 // delta + Gram with the Gram on MFMA (R <= 15); precision selects the delta
@@ -307,15 +314,18 @@ constexpr int kSplitWordsWide = 32;
 bool gram_wide_supported(int R, int order);
 void launch_gram_bf16_wide(int R, const GramArgs &a, hipStream_t s);
 void launch_split_wide(int R, const float *A, uint64_t rows, uint32_t *As, hipStream_t s);
-// Hot outer rows of the wide kernel (ranks whose last tile is not packed:
-// R = 16, 28, 30, 32): GramArgs.hot_slot / hot_tab.  A slot table entry of
-// kNoHotSlot marks a cold row; at most kMaxHotRows slots.
+// Shared factor rows of the wide kernel (ranks whose last tile is not
+// packed: R = 16, 28, 30, 32): GramArgs.hot_kind / hot_slot / hot_tab.  A
+// slot table entry of kNoHotSlot marks a cold row; at most kMaxHotRows slots.
 constexpr uint32_t kNoHotSlot = 0xFFFFFFFFu;
 constexpr int kMaxHotRows = 63;
 bool wide_hot_supported(int R, int order);
-uint64_t hot_table_words(int R, int K); // uint16 words of K slots' tables
-void launch_hot_tables(int R, const float *A_O, const uint32_t *hot_rows, int K, const float *Gt,
-                       uint16_t *out, hipStream_t s);
+uint64_t hot_table_words(int R, uint64_t K); // uint16 words of K tables
+// K tables into out: kind kHotOuter contracts the outer index o with row
+// rows[k] of A (A_O), kind kHotLast contracts jL with row k of A (A_L; rows
+// null).
+void launch_hot_tables(int R, int kind, const float *A, const uint32_t *rows, uint64_t K,
+                       const float *Gt, uint16_t *out, hipStream_t s);
 // The last 16-column output tile of the wide kernel holds RR = R - 16 (MT - 1)
 // columns; when RR <= 8 it packs wide_pack(R) outer indices (rows (o - PO ob,
 // r) with r padded to 16 / PO), one MFMA set per PO outer indices.

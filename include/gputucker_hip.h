@@ -136,11 +136,13 @@ typedef struct gt_partition_info {
   uint64_t host_pageable_bytes; /* out-of-core host data held in pageable memory because the
                                    system granted no more page-locked memory (0 normally);
                                    copies from it are slower and synchronous */
-  uint32_t hot_rows[GT_MAX_ORDER]; /* per mode: rows of its outer factor heavy enough that the
-                                      refined BF16 delta of the wide ranks (order 3, R = 16 or
-                                      25..32, in-core partition) takes them through a core
-                                      contracted with the row, each row's nonzeros grouped by
-                                      them in the stream (0: none) */
+  uint32_t hot_rows[GT_MAX_ORDER]; /* per mode: factor rows the refined BF16 delta of the wide
+                                      ranks (order 3, R = 16 or 25..32, in-core partition) takes
+                                      through a core contracted with the row, each row's
+                                      nonzeros grouped by them in the stream (0: none) */
+  int hot_kind[GT_MAX_ORDER];      /* per mode: 1 the heavy rows of its outer factor, 2 every
+                                      row of its last other factor (rows long enough to hold
+                                      runs of each), 0 none */
 } gt_partition_info;
 
 typedef struct gt_timings {

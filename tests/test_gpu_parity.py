@@ -675,40 +675,51 @@ def test_core_source_variants(N, R, part, paths):
 
 @pytest.mark.parametrize("R", [16, 30, 32])
 def test_wide_hot_outer_rows(R):
-    """Order 3 with mode 0 Zipf(1.5) over 40 rows: a few rows of mode 0 hold
-    at least 16 nonzeros per row of modes 1 and 2, so those modes' refined
-    BF16 wide delta has hot outer rows.  Each row's nonzeros are grouped in
-    the stream by hot outer row (slots by degree, heaviest first, the cold
-    ones last, stably), which a numpy stable sort of the input order pins
-    (P = 1: block order is input order).  A 16-nonzero block sharing one hot
-    row takes the core contracted with it, M[jL][r] = sum_o A_0[h][o]
-    G[o][jL][r]: its deltas (delta.cuh:62-90), B and C (update.cuh:35-101)
-    against the FP64 oracle at the refined BF16 bars, and the fit after one
-    iteration within 1e-4 of the FP64 oracle's.  The FP32 path keeps the
-    plain stream."""
+    """Order 3 with mode 0 Zipf(1.5) over 40 rows.  Modes 1 and 2: a few rows
+    of mode 0 (their outer factor) hold at least 16 nonzeros per row of the
+    mode, so each row's nonzeros are grouped by those hot outer rows (slots
+    by degree, heaviest first, the cold ones last).  Mode 0: most of its
+    nonzeros lie in rows holding at least 16 per row of mode 2 (its last
+    other factor), so each row's nonzeros are grouped by their mode-2 row.
+    Both groupings are stable sorts of the input order, which numpy pins
+    (P = 1: block order is input order).  A 16-nonzero block sharing a hot
+    outer row h takes M_h[jL][r] = sum_o A_0[h][o] G[o][jL][r], one sharing a
+    row l of the last factor N_l[o][r] = sum_jL A_L[l][jL] G[o][jL][r]: the
+    deltas (delta.cuh:62-90), B and C (update.cuh:35-101) against the FP64
+    oracle at the refined BF16 bars, and the fit after one iteration within
+    1e-4 of the FP64 oracle's.  The FP32 path keeps the plain stream."""
     from gputucker_amd import synth
-    print(f"wide hot outer rows: R={R}", flush=True)
-    idx, vals, dims = synth.generate([40, 250, 150], 40000, [("zipf", 1.5), "uniform", "uniform"],
+    print(f"wide shared rows: R={R}", flush=True)
+    nnz = 40000
+    idx, vals, dims = synth.generate([40, 250, 150], nnz, [("zipf", 1.5), "uniform", "uniform"],
                                      "uniform", seed=R)
     part = [1, 1, 1]
     A, G = random_init(R, dims, R)
     e = make(idx, vals, dims, R, part, precision="bf16r", chunk_tiles=1)
     e.set_init(A, G)
-    hot = e.info()["hot_rows"]
+    info = e.info()
+    hot, kind = info["hot_rows"], info["hot_kind"]
     t = oracle.Tensor(idx, vals.astype(np.float64), dims, part, R)
     Ap = [oracle.pad_factor(a.astype(np.float64), t.rows(k)) for k, a in enumerate(A)]
     Gd = G.astype(np.float64)
     lam = gt.default_lambda()
     scale_n = max(1, R ** 2 / 100)
-    outer = {0: 1, 1: 0, 2: 0}
     for m in range(3):
-        O = outer[m]
-        deg = np.bincount(idx[O], minlength=dims[O])
-        cand = [h for h in np.lexsort((np.arange(dims[O]), -deg)) if deg[h] >= 16 * dims[m]][:63]
-        assert hot[m] == len(cand), (m, hot, len(cand))
-        slot = np.full(dims[O], 63, np.int64)
-        slot[cand] = np.arange(len(cand))
-        perm = np.argsort(idx[m].astype(np.int64) * 64 + slot[idx[O]], kind="stable")
+        L = 1 if m == 2 else 2
+        O = 3 - m - L
+        deg_m = np.bincount(idx[m], minlength=dims[m])
+        if 2 * deg_m[deg_m >= 16 * dims[L]].sum() >= nnz:
+            want = (2, dims[L])
+            perm = np.lexsort((np.arange(nnz), idx[L], idx[m]))
+        else:
+            deg = np.bincount(idx[O], minlength=dims[O])
+            cand = [h for h in np.lexsort((np.arange(dims[O]), -deg))
+                    if deg[h] >= 16 * dims[m]][:63]
+            want = (1 if cand else 0, len(cand))
+            slot = np.full(dims[O], 63, np.int64)
+            slot[cand] = np.arange(len(cand))
+            perm = np.argsort(idx[m].astype(np.int64) * 64 + slot[idx[O]], kind="stable")
+        assert (kind[m], hot[m]) == want, (m, kind, hot, want)
         d, idx_s = e.get_stream_delta(m)
         assert np.array_equal(idx_s, idx[:, perm]), m
         ref = oracle.delta(t, Ap, Gd, m)
@@ -721,7 +732,7 @@ def test_wide_hot_outer_rows(R):
         assert (np.abs(Bg - B) / sb).max() < 2e-4, m
         scc = np.abs(Cv).max(axis=1, keepdims=True) + 1e-30
         assert (np.abs(Cg - Cv) / scc).max() < 2e-4, m
-    assert hot[0] == 0 and hot[1] >= 2 and hot[2] >= 2, hot
+    assert kind.tolist() == [2, 1, 1] and hot[1] >= 2 and hot[2] >= 2, (kind, hot)
     e.close()
     f = make(idx, vals, dims, R, part, precision="fp32", chunk_tiles=1)
     assert f.info()["hot_rows"].tolist() == [0, 0, 0]
