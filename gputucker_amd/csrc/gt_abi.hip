@@ -307,12 +307,11 @@ struct gt_ctx {
   DevBuf<uint64_t> row_ptr[GT_MAX_ORDER];
   DevBuf<uint32_t> where[GT_MAX_ORDER];
   uint64_t rows[GT_MAX_ORDER] = {};
-  // hot outer rows of mode n's wide refined BF16 delta (in-core partitions):
-  // the rows of the outer factor holding so many nonzeros that mode n's rows
-  // carry long runs of them (each row's nonzeros are grouped by them in the
-  // stream); their slot per outer row, the rows, and the contracted-core
-  // tables rebuilt at each mode update
-  int hot_kind[GT_MAX_ORDER] = {}; // kHotOuter, kHotLast or 0
+  // shared factor rows of mode n's wide refined BF16 delta (in-core
+  // partitions, group_hot_rows): the kind, the number of contracted-core
+  // tables (rebuilt at each mode update), and for kHotOuter the slot of each
+  // outer row and the hot rows
+  int hot_kind[GT_MAX_ORDER] = {};   // kHotOuter, kHotLast or 0
   uint64_t hot_k[GT_MAX_ORDER] = {}; // tables
   DevBuf<uint32_t> hot_slot[GT_MAX_ORDER];
   DevBuf<uint32_t> hot_rows[GT_MAX_ORDER];
